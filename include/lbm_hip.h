@@ -1,0 +1,177 @@
+/*
+ * lbm_hip.h -- C ABI of the MI355X collide-stream time step.
+ *
+ * One handle (lbm_domain) is one single-device domain: the ghosted lattice of
+ * the reference's Domain<M> (include/domain.h:10-56) with its boundary
+ * handlers, held in HBM of one GPU and advanced by hand-written HIP kernels.
+ * Every entry point returns 0 on success or a negative LBM_ERR_* code; the
+ * message of the last failure on the calling thread is lbm_last_error().
+ * Calls on one handle must come from one host thread at a time. Work is
+ * enqueued on the handle's own HIP stream; functions that return data to the
+ * host synchronise that stream. No pointer passed in is retained after a call
+ * returns.
+ *
+ * Coordinates are the reference's ghosted coordinates: interior cells are
+ * 1..xl, 1..yl, 1..zl; 0 and l+1 are the ghost layer (include/domain.hpp:67-70).
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_HIP_ABI_VERSION 1
+
+/* Status codes. */
+#define LBM_OK 0
+#define LBM_ERR_ARG (-1)         /* invalid argument */
+#define LBM_ERR_STATE (-2)       /* call not valid in the handle's state */
+#define LBM_ERR_HIP (-3)         /* HIP runtime error */
+#define LBM_ERR_ALLOC (-4)       /* device or host allocation failed */
+#define LBM_ERR_UNSUPPORTED (-5) /* combination not implemented */
+#define LBM_ERR_GEOMETRY (-6)    /* scenario geometry rejected at finalize */
+
+/* Storage of the distribution fields.
+ * F64  : the reference's fp64 pdfs and operation order (parity mode).
+ * F32, F16, BF16 : shifted values f - w stored in that format, fp32 math.
+ * F64S : shifted values in fp64 with fp64 math (reference for the others). */
+enum lbm_storage {
+    LBM_STORAGE_F64 = 0,
+    LBM_STORAGE_F32 = 1,
+    LBM_STORAGE_F16 = 2,
+    LBM_STORAGE_BF16 = 3,
+    LBM_STORAGE_F64S = 4
+};
+
+/* Boundary conditions: the reference's NonFluidCollision subclasses
+ * (include/boundary.h:11-70). params[4] = {vx, vy, vz, rho}:
+ *   NOSLIP      -                          NoSlipBoundary(domain)
+ *   MOVINGWALL  wall velocity vx,vy,vz      MovingWallBoundary(domain, u_w)
+ *   FREESLIP    -                          FreeSlipBoundary(domain)
+ *   OUTFLOW     rho = reference density     OutflowBoundary(domain, rho_ref)
+ *   INFLOW      vx,vy,vz, rho = ref density InflowBoundary(domain, u_in, rho_ref)
+ *   PRESSURE    rho = input density         PressureBoundary(domain, rho_in) */
+enum lbm_bc {
+    LBM_BC_NOSLIP = 1,
+    LBM_BC_MOVINGWALL = 2,
+    LBM_BC_FREESLIP = 3,
+    LBM_BC_OUTFLOW = 4,
+    LBM_BC_INFLOW = 5,
+    LBM_BC_PRESSURE = 6
+};
+
+typedef struct lbm_desc {
+    int q;             /* 15, 19 or 27: lbm::model::d3q15/d3q19/d3q27 (include/model.h) */
+    int storage;       /* enum lbm_storage */
+    int device;        /* HIP device ordinal */
+    int periodic[3];   /* extension: periodic x, y, z (the reference has none) */
+    uint64_t xl, yl, zl; /* interior lengths, Domain(xl, yl, zl, ...) */
+    double tau;        /* BGKCollision(tau), include/collision.hpp:55-58 */
+    double origin[3];  /* Domain origin (VTK output only) */
+    double spacing[3]; /* Domain spacing (VTK output only) */
+} lbm_desc;
+
+typedef struct lbm_domain lbm_domain;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lbm_last_error(void);
+int lbm_abi_version(void);
+
+/* Lattice tables of lbm::model::d3q15/d3q19/d3q27 (include/model.h:13-139)
+ * and lbm::C_S (include/lbmdefinitions.h:47) as this library uses them:
+ * c has 3*q ints (x, y, z per direction), w has q doubles; any output may be
+ * NULL. Needs no GPU. */
+int lbm_lattice(int q, int* c, double* w, double* cs);
+
+/* Fills defaults: q=19, F64, device 0, no periodic axes, tau=1, origin 0,
+ * spacing 1, lengths 0. */
+int lbm_desc_init(lbm_desc* desc);
+
+/* Replaces Domain<M>(xl, yl, zl, collision, origin, spacing)
+ * (include/domain.hpp:87-98) plus BGKCollision<M>(tau) (src/main.cpp:36):
+ * every cell of the ghosted box starts as a BGK fluid cell holding the
+ * lattice weights (include/cell.hpp:9-15). */
+int lbm_domain_create(const lbm_desc* desc, lbm_domain** out);
+int lbm_domain_destroy(lbm_domain* d);
+
+/* Replaces Domain::setBoundaryCondition(BoundaryKeeper::get_collision<K>(...),
+ * x0, xE, y0, yE, z0, zE) (include/domain.hpp:175-194,
+ * include/io/scenario.h:99-127): inclusive box in ghosted coordinates, later
+ * calls override earlier ones cell by cell. */
+int lbm_set_boundary_box(lbm_domain* d, int kind, const double params[4], uint64_t x0,
+                         uint64_t xe, uint64_t y0, uint64_t ye, uint64_t z0, uint64_t ze);
+
+/* Replaces the mask part of io::read_vtk_point_file (include/io/vtk.hpp:137-149):
+ * fluid_mask has xl*yl*zl bytes, x fastest; interior cells with 0 become
+ * NoSlip cells. */
+int lbm_set_solid_mask(lbm_domain* d, const uint8_t* fluid_mask);
+
+/* Extension: Brinkman porosity alpha in [0,1] per interior cell (xl*yl*zl,
+ * x fastest, NULL removes it) and body acceleration g[3]. The equilibrium
+ * velocity becomes (1 - alpha) * (u + tau * g). Shifted storage modes only. */
+int lbm_set_brinkman(lbm_domain* d, const double* alpha, const double g[3]);
+
+/* Replaces Domain::set_nonfluid_cells_nullcollide (include/domain.hpp:101-113,
+ * called at src/main.cpp:44): freezes the geometry, builds the device cell
+ * kinds and the near-boundary cell list. Rejects (LBM_ERR_GEOMETRY) a domain
+ * in which an interior fluid cell would pull from a ghost cell that no
+ * boundary covers. Must precede lbm_step. */
+int lbm_finalize(lbm_domain* d);
+
+/* Replaces nsteps iterations of { stream(); swap(); collide(); }
+ * (src/main.cpp:50-52, include/domain.hpp:116-172). Asynchronous. */
+int lbm_step(lbm_domain* d, uint64_t nsteps);
+/* Waits for the handle's stream. */
+int lbm_sync(lbm_domain* d);
+
+/* Replaces reads of cell(x,y,z)[q] over the whole ghosted collide_field
+ * (include/cell.hpp:24-33): out has (xl+2)(yl+2)(zl+2)*q doubles in the
+ * reference order idx(x,y,z)*q + q. Non-fluid cells hold the values the
+ * reference's boundary pass leaves there. */
+int lbm_get_pdf(lbm_domain* d, double* out);
+/* Replaces writes of cell(x,y,z)[q] over the whole ghosted collide_field;
+ * the next step pulls the written values from non-fluid cells as stored. */
+int lbm_set_pdf(lbm_domain* d, const double* in);
+/* Extension: every interior fluid cell <- feq(rho, u) from x-fastest
+ * interior arrays rho[xl*yl*zl], u[3*xl*yl*zl] (host pointers). */
+int lbm_init_equilibrium(lbm_domain* d, const double* rho, const double* u);
+
+/* Replaces Cell::density() / Cell::velocity(density) of every interior cell
+ * as the VTK writer evaluates them (include/io/vtk.hpp:62-73): rho has
+ * xl*yl*zl doubles, u 3*xl*yl*zl, x fastest. Either may be NULL. */
+int lbm_moments(lbm_domain* d, double* rho, double* u);
+/* Extension: fp64 sums over interior fluid cells of rho and of the momentum
+ * (x, y, z), reduced on the device. */
+int lbm_conservation(lbm_domain* d, double out[4]);
+
+/* Timing: runs nsteps steps with HIP events on the handle's stream around
+ * every kernel launch. out[0] = total ms, out[1] = ms in the bulk kernel,
+ * out[2] = ms in the near-boundary kernel, out[3] = number of steps. */
+int lbm_time_steps(lbm_domain* d, uint64_t nsteps, double out[4]);
+
+/* Layout and size information. */
+typedef struct lbm_info {
+    uint64_t ncells_interior;  /* xl*yl*zl */
+    uint64_t ncells_fluid;     /* interior fluid cells */
+    uint64_t ncells_near;      /* fluid cells with a non-fluid source */
+    uint64_t bytes_per_value;  /* storage element size */
+    uint64_t field_bytes;      /* bytes of one distribution buffer */
+    uint64_t device_bytes;     /* all device memory of the handle */
+    int vec;                   /* cells per thread of the bulk kernel */
+    int block_x, block_y;      /* bulk kernel block shape */
+    int nconds;                /* distinct boundary conditions */
+} lbm_info;
+int lbm_get_info(lbm_domain* d, lbm_info* info);
+
+/* Tuning: cells per thread (must be one the mode supports; 0 = default) and
+ * threads along x per block (64, 128 or 256; 0 = automatic). */
+int lbm_set_tuning(lbm_domain* d, int vec, int block_x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_HIP_H */

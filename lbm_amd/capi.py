@@ -1,0 +1,227 @@
+"""ctypes binding of the C ABI in include/lbm_hip.h (liblbm_hip.so).
+
+This is the Python face of the drop-in boundary: the same entry points a C or
+C++ caller uses, wrapped in a small `Domain` class that mirrors the reference's
+`Domain` lifecycle (construct, setBoundaryCondition..., nullcollide/finalize,
+stream/swap/collide steps, cell/moment reads). The library must be built
+(`python -m lbm_amd.build`); there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, byref, c_char_p, c_double, c_int, c_uint8, c_uint64, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "liblbm_hip.so"
+
+STORAGE = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3, "f64s": 4}
+STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f64s": 8}
+
+NOSLIP, MOVINGWALL, FREESLIP, OUTFLOW, INFLOW, PRESSURE = 1, 2, 3, 4, 5, 6
+BC_KINDS = {"noslip": NOSLIP, "movingwall": MOVINGWALL, "freeslip": FREESLIP,
+            "outflow": OUTFLOW, "inflow": INFLOW, "pressure": PRESSURE}
+
+ERRORS = {-1: "LBM_ERR_ARG", -2: "LBM_ERR_STATE", -3: "LBM_ERR_HIP", -4: "LBM_ERR_ALLOC",
+          -5: "LBM_ERR_UNSUPPORTED", -6: "LBM_ERR_GEOMETRY"}
+
+
+class LbmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("q", c_int), ("storage", c_int), ("device", c_int), ("periodic", c_int * 3),
+                ("xl", c_uint64), ("yl", c_uint64), ("zl", c_uint64), ("tau", c_double),
+                ("origin", c_double * 3), ("spacing", c_double * 3)]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("ncells_interior", c_uint64), ("ncells_fluid", c_uint64),
+                ("ncells_near", c_uint64), ("bytes_per_value", c_uint64),
+                ("field_bytes", c_uint64), ("device_bytes", c_uint64), ("vec", c_int),
+                ("block_x", c_int), ("block_y", c_int), ("nconds", c_int)]
+
+
+# (name, restype, argtypes) of every exported entry point in include/lbm_hip.h
+SIGNATURES = [
+    ("lbm_last_error", c_char_p, []),
+    ("lbm_abi_version", c_int, []),
+    ("lbm_lattice", c_int, [c_int, POINTER(c_int), POINTER(c_double), POINTER(c_double)]),
+    ("lbm_desc_init", c_int, [POINTER(Desc)]),
+    ("lbm_domain_create", c_int, [POINTER(Desc), POINTER(c_void_p)]),
+    ("lbm_domain_destroy", c_int, [c_void_p]),
+    ("lbm_set_boundary_box", c_int, [c_void_p, c_int, POINTER(c_double)] + [c_uint64] * 6),
+    ("lbm_set_solid_mask", c_int, [c_void_p, POINTER(c_uint8)]),
+    ("lbm_set_brinkman", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+    ("lbm_finalize", c_int, [c_void_p]),
+    ("lbm_step", c_int, [c_void_p, c_uint64]),
+    ("lbm_sync", c_int, [c_void_p]),
+    ("lbm_get_pdf", c_int, [c_void_p, POINTER(c_double)]),
+    ("lbm_set_pdf", c_int, [c_void_p, POINTER(c_double)]),
+    ("lbm_init_equilibrium", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+    ("lbm_moments", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+    ("lbm_conservation", c_int, [c_void_p, POINTER(c_double)]),
+    ("lbm_time_steps", c_int, [c_void_p, c_uint64, POINTER(c_double)]),
+    ("lbm_get_info", c_int, [c_void_p, POINTER(Info)]),
+    ("lbm_set_tuning", c_int, [c_void_p, c_int, c_int]),
+]
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads liblbm_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m lbm_amd.build`")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise LbmError(rc, load().lbm_last_error().decode())
+
+
+def lattice(q: int):
+    """(c[Q,3], w[Q], cs) as the library holds them; needs no GPU."""
+    c = np.zeros((q, 3), dtype=np.int32)
+    w = np.zeros(q)
+    cs = c_double()
+    _check(load().lbm_lattice(q, c.ctypes.data_as(POINTER(c_int)),
+                              w.ctypes.data_as(POINTER(c_double)), byref(cs)))
+    return c.astype(np.int64), w, cs.value
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+class Domain:
+    """One single-device domain (include/lbm_hip.h: lbm_domain)."""
+
+    def __init__(self, xl: int, yl: int, zl: int, tau: float, q: int = 19, storage: str = "f64",
+                 periodic=(False, False, False), device: int = 0,
+                 origin=(0.0, 0.0, 0.0), spacing=(1.0, 1.0, 1.0)):
+        lib = load()
+        d = Desc()
+        _check(lib.lbm_desc_init(byref(d)))
+        d.q, d.storage, d.device = q, STORAGE[storage], device
+        for k in range(3):
+            d.periodic[k] = 1 if periodic[k] else 0
+            d.origin[k] = origin[k]
+            d.spacing[k] = spacing[k]
+        d.xl, d.yl, d.zl, d.tau = xl, yl, zl, tau
+        h = c_void_p()
+        _check(lib.lbm_domain_create(byref(d), byref(h)))
+        self._h = h
+        self.q, self.xl, self.yl, self.zl, self.tau = q, xl, yl, zl, tau
+        self.storage = storage
+        self.periodic = tuple(bool(p) for p in periodic)
+
+    # -- lifecycle -----------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            load().lbm_domain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- geometry ------------------------------------------------------------
+    def set_boundary_box(self, kind, params, x0, xe, y0, ye, z0, ze) -> None:
+        if isinstance(kind, str):
+            kind = BC_KINDS[kind]
+        p = np.ascontiguousarray(params if params is not None else [0, 0, 0, 1], dtype=np.float64)
+        _check(load().lbm_set_boundary_box(self._h, kind, _dptr(p), x0, xe, y0, ye, z0, ze))
+
+    def set_solid_mask(self, fluid_mask: np.ndarray) -> None:
+        m = np.ascontiguousarray(fluid_mask, dtype=np.uint8).reshape(-1)
+        assert m.size == self.xl * self.yl * self.zl
+        _check(load().lbm_set_solid_mask(self._h, m.ctypes.data_as(POINTER(c_uint8))))
+
+    def set_brinkman(self, alpha, g=(0.0, 0.0, 0.0)) -> None:
+        gv = np.ascontiguousarray(g, dtype=np.float64)
+        if alpha is None:
+            _check(load().lbm_set_brinkman(self._h, None, _dptr(gv)))
+            return
+        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1)
+        assert a.size == self.xl * self.yl * self.zl
+        _check(load().lbm_set_brinkman(self._h, _dptr(a), _dptr(gv)))
+
+    def finalize(self) -> None:
+        _check(load().lbm_finalize(self._h))
+
+    # -- time stepping -------------------------------------------------------
+    def step(self, n: int = 1) -> None:
+        _check(load().lbm_step(self._h, n))
+
+    def sync(self) -> None:
+        _check(load().lbm_sync(self._h))
+
+    def time_steps(self, n: int) -> dict:
+        out = np.zeros(4)
+        _check(load().lbm_time_steps(self._h, n, _dptr(out)))
+        return {"total_ms": out[0], "bulk_ms": out[1], "near_ms": out[2], "steps": int(out[3])}
+
+    # -- fields --------------------------------------------------------------
+    @property
+    def ghost_shape(self):
+        return (self.zl + 2, self.yl + 2, self.xl + 2)
+
+    def get_pdf(self) -> np.ndarray:
+        """Ghosted collide field as (zl+2, yl+2, xl+2, Q) doubles."""
+        out = np.empty(self.ghost_shape + (self.q,), dtype=np.float64)
+        _check(load().lbm_get_pdf(self._h, _dptr(out)))
+        return out
+
+    def set_pdf(self, f: np.ndarray) -> None:
+        a = np.ascontiguousarray(f, dtype=np.float64)
+        assert a.size == np.prod(self.ghost_shape) * self.q
+        _check(load().lbm_set_pdf(self._h, _dptr(a)))
+
+    def init_equilibrium(self, rho: np.ndarray, u: np.ndarray) -> None:
+        r = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        n = self.xl * self.yl * self.zl
+        assert r.size == n and v.size == 3 * n
+        _check(load().lbm_init_equilibrium(self._h, _dptr(r), _dptr(v)))
+
+    def moments(self):
+        """(rho, u): interior (zl, yl, xl) and (zl, yl, xl, 3) doubles."""
+        rho = np.empty((self.zl, self.yl, self.xl), dtype=np.float64)
+        u = np.empty((self.zl, self.yl, self.xl, 3), dtype=np.float64)
+        _check(load().lbm_moments(self._h, _dptr(rho), _dptr(u)))
+        return rho, u
+
+    def conservation(self) -> np.ndarray:
+        out = np.zeros(4)
+        _check(load().lbm_conservation(self._h, _dptr(out)))
+        return out
+
+    def info(self) -> dict:
+        i = Info()
+        _check(load().lbm_get_info(self._h, byref(i)))
+        return {k: getattr(i, k) for k, _ in Info._fields_}
+
+    def set_tuning(self, vec: int = 0, block_x: int = 0) -> None:
+        _check(load().lbm_set_tuning(self._h, vec, block_x))

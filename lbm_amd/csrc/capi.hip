@@ -1,0 +1,724 @@
+// C ABI of the MI355X collide-stream step (include/lbm_hip.h).
+//
+// Host side of one domain: the handler map of the reference's ghosted box
+// (one boundary-condition object per setBoundaryCondition call, last call
+// wins per cell), its translation into device cell kinds at finalize, the
+// two distribution buffers and the per-step launch of the kernels of the
+// domain's (lattice, storage) pair.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbm_hip.h"
+#include "domain_view.hpp"
+#include "lattice.hpp"
+#include "ops.hpp"
+#include "storage.hpp"
+
+using namespace lbm_amd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(LBM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct HostCond {
+    int kind;
+    double vel[3];
+    double rho;
+};
+
+// Lattice tables as plain arrays for host code.
+struct HostLattice {
+    int Q = 0;
+    int c[27][3];
+    double w[27];
+};
+
+template <class L>
+HostLattice make_lattice() {
+    HostLattice h;
+    h.Q = L::Q;
+    for (int q = 0; q < L::Q; ++q) {
+        h.c[q][0] = L::cx(q);
+        h.c[q][1] = L::cy(q);
+        h.c[q][2] = L::cz(q);
+        h.w[q] = weight<L>(q);
+    }
+    return h;
+}
+
+HostLattice lattice_for(int q) {
+    if (q == 15) return make_lattice<D3Q15>();
+    if (q == 19) return make_lattice<D3Q19>();
+    return make_lattice<D3Q27>();
+}
+
+size_t elem_size(Mode m) {
+    switch (m) {
+        case Mode::EXACT64:
+        case Mode::F64S: return 8;
+        case Mode::F32: return 4;
+        default: return 2;
+    }
+}
+
+int64_t roundup(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+// Reference equilibrium (include/collision.hpp:34-51) on the host, for the
+// constant Inflow table.
+#pragma clang fp contract(off)
+void host_feq(const HostLattice& lat, double density, const double* v, double* feq) {
+    const double cs = kCS;
+    for (int q = 0; q < lat.Q; ++q) {
+        double c_dot_u = 0.0;
+        double u_dot_u = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            c_dot_u += static_cast<double>(lat.c[q][d]) * v[d];
+            u_dot_u += v[d] * v[d];
+        }
+        feq[q] = lat.w[q] * density *
+                 (1 + c_dot_u / (cs * cs) + (c_dot_u) * (c_dot_u) / (2 * cs * cs * cs * cs) -
+                  u_dot_u / (2 * cs * cs));
+    }
+}
+#pragma clang fp contract(on)
+
+template <class T>
+__global__ void k_fill_planes(T* buf, int64_t plane, int q, const double* values) {
+    const int64_t n = plane * q;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        buf[i] = static_cast<T>(values[i / plane]);
+}
+
+}  // namespace
+
+struct lbm_domain {
+    lbm_desc desc{};
+    Mode mode = Mode::EXACT64;
+    const ModeOps* ops = nullptr;
+    HostLattice lat;
+    int Q = 0;
+    int nx = 0, ny = 0, nz = 0;
+    int64_t ncell = 0;
+    FieldLayout L{};
+    size_t esize = 0;
+    size_t field_bytes = 0;
+    void* buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint8_t* d_kind = nullptr;
+    CellCoord* d_blist = nullptr;
+    int64_t nb = 0;
+    Cond* d_conds = nullptr;
+    int nconds_dev = 0;
+    void* d_alpha = nullptr;
+    double g[3] = {0, 0, 0};
+    std::vector<uint16_t> handler;  // per ghosted cell: 0 fluid, else 1 + index in conds
+    std::vector<HostCond> conds;
+    int mask_cond = -1;
+    std::vector<double> h_alpha;  // interior, x fastest (empty: none)
+    bool finalized = false;
+    bool materialised = true;
+    hipStream_t stream = nullptr;
+    StepPlan plan{};
+    int tune_vec = 0, tune_bx = 0;
+    double* d_partial = nullptr;
+    double* d_out4 = nullptr;
+    int nred = 0;
+    uint64_t n_fluid = 0;
+    size_t device_bytes = 0;
+
+    int64_t hidx(int64_t x, int64_t y, int64_t z) const { return x + nx * (y + ny * z); }
+    bool interior(int64_t x, int64_t y, int64_t z) const {
+        return x >= 1 && x <= L.xl && y >= 1 && y <= L.yl && z >= 1 && z <= L.zl;
+    }
+    int64_t wrapc(int64_t v, int axis) const {
+        const int64_t l = axis == 0 ? L.xl : (axis == 1 ? L.yl : L.zl);
+        if (!L.per[axis]) return v;
+        if (v < 1) return v + l;
+        if (v > l) return v - l;
+        return v;
+    }
+    StepArgs args(const void* src, void* dst, int mat) const {
+        StepArgs a{};
+        a.L = L;
+        a.src = src;
+        a.dst = dst;
+        a.kind = d_kind;
+        a.blist = d_blist;
+        a.nb = nb;
+        a.conds = d_conds;
+        a.materialised = mat;
+        a.tau = desc.tau;
+        a.omega = 1.0 / desc.tau;
+        a.alpha = d_alpha;
+        a.g[0] = g[0];
+        a.g[1] = g[1];
+        a.g[2] = g[2];
+        return a;
+    }
+};
+
+namespace {
+
+int check_handle(lbm_domain* d) {
+    if (!d) return fail(LBM_ERR_ARG, "null domain handle");
+    hipError_t e = hipSetDevice(d->desc.device);
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int make_plan(lbm_domain* d) {
+    const ModeOps* o = d->ops;
+    int V = d->tune_vec ? d->tune_vec : o->default_vec;
+    bool ok = false;
+    for (int i = 0; i < 4 && o->vecs[i]; ++i) ok |= (o->vecs[i] == V);
+    if (!ok) return fail(LBM_ERR_ARG, "vector width " + std::to_string(V) + " not built for this mode");
+    StepPlan& p = d->plan;
+    p.V = V;
+    const int nvx = (d->L.xl + V - 1) / V;
+    int bx = d->tune_bx;
+    if (!bx) {
+        bx = 1;
+        while (bx < nvx && bx < 256) bx *= 2;
+    }
+    if (bx < 1 || bx > 256 || (256 % bx)) return fail(LBM_ERR_ARG, "block_x must divide 256");
+    p.BX = bx;
+    p.BY = 256 / bx;
+    p.nbx = (nvx + p.BX - 1) / p.BX;
+    p.nby = (d->L.yl + p.BY - 1) / p.BY;
+    p.nblocks_bulk = static_cast<int64_t>(p.nbx) * p.nby * d->L.zl;
+    if (p.nblocks_bulk > 0x7fffffffLL) return fail(LBM_ERR_UNSUPPORTED, "grid too large");
+    return LBM_OK;
+}
+
+void free_device(lbm_domain* d) {
+    (void)hipFree(d->buf[0]);
+    (void)hipFree(d->buf[1]);
+    (void)hipFree(d->d_kind);
+    (void)hipFree(d->d_blist);
+    (void)hipFree(d->d_conds);
+    (void)hipFree(d->d_alpha);
+    (void)hipFree(d->d_partial);
+    (void)hipFree(d->d_out4);
+    d->buf[0] = d->buf[1] = nullptr;
+    d->d_kind = nullptr;
+    d->d_blist = nullptr;
+    d->d_conds = nullptr;
+    d->d_alpha = nullptr;
+    d->d_partial = nullptr;
+    d->d_out4 = nullptr;
+}
+
+int fill_initial(lbm_domain* d, void* buf) {
+    // EXACT64 holds the weights; shifted modes hold w - w = 0.
+    if (d->mode != Mode::EXACT64) {
+        HIPCHK(hipMemsetAsync(buf, 0, d->field_bytes, d->stream));
+        return LBM_OK;
+    }
+    double* dv = nullptr;
+    HIPCHK(hipMalloc(&dv, sizeof(double) * d->Q));
+    HIPCHK(hipMemcpyAsync(dv, d->lat.w, sizeof(double) * d->Q, hipMemcpyHostToDevice, d->stream));
+    hipLaunchKernelGGL(k_fill_planes<double>, dim3(4096), dim3(256), 0, d->stream,
+                       static_cast<double*>(buf), d->L.plane, d->Q, dv);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(d->stream));
+    HIPCHK(hipFree(dv));
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbm_last_error(void) { return g_err.c_str(); }
+int lbm_abi_version(void) { return LBM_HIP_ABI_VERSION; }
+
+int lbm_lattice(int q, int* c, double* w, double* cs) {
+    if (q != 15 && q != 19 && q != 27) return fail(LBM_ERR_ARG, "q must be 15, 19 or 27");
+    const HostLattice lat = lattice_for(q);
+    for (int i = 0; i < q; ++i) {
+        if (c)
+            for (int k = 0; k < 3; ++k) c[3 * i + k] = lat.c[i][k];
+        if (w) w[i] = lat.w[i];
+    }
+    if (cs) *cs = kCS;
+    return LBM_OK;
+}
+
+int lbm_desc_init(lbm_desc* desc) {
+    if (!desc) return fail(LBM_ERR_ARG, "null desc");
+    std::memset(desc, 0, sizeof(*desc));
+    desc->q = 19;
+    desc->storage = LBM_STORAGE_F64;
+    desc->tau = 1.0;
+    desc->spacing[0] = desc->spacing[1] = desc->spacing[2] = 1.0;
+    return LBM_OK;
+}
+
+int lbm_domain_create(const lbm_desc* desc, lbm_domain** out) {
+    if (!desc || !out) return fail(LBM_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (desc->q != 15 && desc->q != 19 && desc->q != 27)
+        return fail(LBM_ERR_ARG, "q must be 15, 19 or 27");
+    if (desc->storage < 0 || desc->storage > 4) return fail(LBM_ERR_ARG, "unknown storage mode");
+    const uint64_t maxl = 1u << 20;
+    if (desc->xl < 1 || desc->yl < 1 || desc->zl < 1 || desc->xl > maxl || desc->yl > maxl ||
+        desc->zl > maxl)
+        return fail(LBM_ERR_ARG, "lengths must be in [1, 2^20]");
+    if (!(desc->tau > 0.5) || !std::isfinite(desc->tau))
+        return fail(LBM_ERR_ARG, "tau must be > 0.5");
+    hipError_t e = hipSetDevice(desc->device);
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+
+    lbm_domain* d = new lbm_domain();
+    d->desc = *desc;
+    d->mode = static_cast<Mode>(desc->storage);
+    d->ops = mode_ops(desc->q, d->mode);
+    if (!d->ops) {
+        delete d;
+        return fail(LBM_ERR_UNSUPPORTED, "no kernels for this lattice/storage");
+    }
+    d->lat = lattice_for(desc->q);
+    d->Q = desc->q;
+    d->nx = static_cast<int>(desc->xl + 2);
+    d->ny = static_cast<int>(desc->yl + 2);
+    d->nz = static_cast<int>(desc->zl + 2);
+    d->ncell = static_cast<int64_t>(d->nx) * d->ny * d->nz;
+    d->esize = elem_size(d->mode);
+    const int64_t A = 128 / static_cast<int64_t>(d->esize);
+    FieldLayout& L = d->L;
+    L.off = A - 1;
+    L.px = roundup(d->nx, A);
+    L.pxy = L.px * d->ny;
+    L.plane = roundup(L.off + L.pxy * d->nz + 16, A);
+    L.xl = static_cast<int>(desc->xl);
+    L.yl = static_cast<int>(desc->yl);
+    L.zl = static_cast<int>(desc->zl);
+    for (int k = 0; k < 3; ++k) L.per[k] = desc->periodic[k] ? 1 : 0;
+    d->field_bytes = static_cast<size_t>(L.plane) * d->Q * d->esize;
+    d->handler.assign(static_cast<size_t>(d->ncell), 0);
+
+    auto bail = [&](int code, const std::string& msg) {
+        free_device(d);
+        if (d->stream) (void)hipStreamDestroy(d->stream);
+        delete d;
+        return fail(code, msg);
+    };
+    if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(LBM_ERR_HIP, "hipStreamCreate failed");
+    for (int k = 0; k < 2; ++k)
+        if (hipMalloc(&d->buf[k], d->field_bytes) != hipSuccess)
+            return bail(LBM_ERR_ALLOC, "hipMalloc of a distribution buffer failed");
+    if (hipMalloc(&d->d_kind, static_cast<size_t>(L.plane)) != hipSuccess)
+        return bail(LBM_ERR_ALLOC, "hipMalloc of the kind array failed");
+    d->device_bytes = 2 * d->field_bytes + static_cast<size_t>(L.plane);
+    for (int k = 0; k < 2; ++k) {
+        int rc = fill_initial(d, d->buf[k]);
+        if (rc) return bail(rc, g_err);
+    }
+    if (hipStreamSynchronize(d->stream) != hipSuccess) return bail(LBM_ERR_HIP, "initial fill failed");
+    *out = d;
+    return LBM_OK;
+}
+
+int lbm_domain_destroy(lbm_domain* d) {
+    if (!d) return LBM_OK;
+    (void)hipSetDevice(d->desc.device);
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    free_device(d);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+    return LBM_OK;
+}
+
+int lbm_set_boundary_box(lbm_domain* d, int kind, const double params[4], uint64_t x0,
+                         uint64_t xe, uint64_t y0, uint64_t ye, uint64_t z0, uint64_t ze) {
+    if (!d) return fail(LBM_ERR_ARG, "null domain handle");
+    if (d->finalized) return fail(LBM_ERR_STATE, "geometry is frozen after lbm_finalize");
+    if (kind < LBM_BC_NOSLIP || kind > LBM_BC_PRESSURE)
+        return fail(LBM_ERR_ARG, "unknown boundary kind " + std::to_string(kind));
+    // include/domain.hpp:180-181
+    if (xe < x0 || ye < y0 || ze < z0) return fail(LBM_ERR_ARG, "box end before box start");
+    if (xe > static_cast<uint64_t>(d->nx - 1) || ye > static_cast<uint64_t>(d->ny - 1) ||
+        ze > static_cast<uint64_t>(d->nz - 1))
+        return fail(LBM_ERR_ARG, "box exceeds the ghosted domain");
+    if (d->conds.size() >= 0xFFFE) return fail(LBM_ERR_ARG, "too many boundary conditions");
+    HostCond c{};
+    c.kind = kind;
+    c.rho = 1.0;
+    if (params) {
+        c.vel[0] = params[0];
+        c.vel[1] = params[1];
+        c.vel[2] = params[2];
+        c.rho = params[3];
+    }
+    d->conds.push_back(c);
+    const uint16_t h = static_cast<uint16_t>(d->conds.size());
+    for (uint64_t z = z0; z <= ze; ++z)
+        for (uint64_t y = y0; y <= ye; ++y)
+            for (uint64_t x = x0; x <= xe; ++x) d->handler[d->hidx(x, y, z)] = h;
+    return LBM_OK;
+}
+
+int lbm_set_solid_mask(lbm_domain* d, const uint8_t* fluid_mask) {
+    if (!d || !fluid_mask) return fail(LBM_ERR_ARG, "null argument");
+    if (d->finalized) return fail(LBM_ERR_STATE, "geometry is frozen after lbm_finalize");
+    if (d->mask_cond < 0) {
+        HostCond c{};
+        c.kind = LBM_BC_NOSLIP;
+        c.rho = 1.0;
+        d->conds.push_back(c);
+        d->mask_cond = static_cast<int>(d->conds.size()) - 1;
+    }
+    const uint16_t h = static_cast<uint16_t>(d->mask_cond + 1);
+    int64_t i = 0;
+    for (int z = 1; z <= d->L.zl; ++z)
+        for (int y = 1; y <= d->L.yl; ++y)
+            for (int x = 1; x <= d->L.xl; ++x, ++i)
+                if (!fluid_mask[i]) d->handler[d->hidx(x, y, z)] = h;
+    return LBM_OK;
+}
+
+int lbm_set_brinkman(lbm_domain* d, const double* alpha, const double g[3]) {
+    if (!d) return fail(LBM_ERR_ARG, "null domain handle");
+    if (d->finalized) return fail(LBM_ERR_STATE, "call lbm_set_brinkman before lbm_finalize");
+    if (d->mode == Mode::EXACT64)
+        return fail(LBM_ERR_UNSUPPORTED, "Brinkman forcing needs a shifted storage mode");
+    const int64_t n = static_cast<int64_t>(d->L.xl) * d->L.yl * d->L.zl;
+    if (alpha)
+        d->h_alpha.assign(alpha, alpha + n);
+    else
+        d->h_alpha.clear();
+    for (int k = 0; k < 3; ++k) d->g[k] = g ? g[k] : 0.0;
+    return LBM_OK;
+}
+
+int lbm_finalize(lbm_domain* d) {
+    if (int rc = check_handle(d)) return rc;
+    if (d->finalized) return fail(LBM_ERR_STATE, "already finalized");
+    const HostLattice& lat = d->lat;
+    const int Q = d->Q;
+    const FieldLayout& L = d->L;
+    auto is_fluid = [&](int64_t x, int64_t y, int64_t z) { return d->handler[d->hidx(x, y, z)] == 0; };
+
+    // Reject pulls from unassigned ghost cells, and FreeSlip chains that would
+    // consult one (the reference treats such a ghost as a BGK cell that is
+    // collided but never streamed into).
+    for (int z = 1; z <= L.zl; ++z)
+        for (int y = 1; y <= L.yl; ++y)
+            for (int x = 1; x <= L.xl; ++x) {
+                if (!is_fluid(x, y, z)) continue;
+                for (int q = 0; q < Q; ++q) {
+                    const int64_t sx = d->wrapc(x - lat.c[q][0], 0);
+                    const int64_t sy = d->wrapc(y - lat.c[q][1], 1);
+                    const int64_t sz = d->wrapc(z - lat.c[q][2], 2);
+                    const uint16_t h = d->handler[d->hidx(sx, sy, sz)];
+                    if (!d->interior(sx, sy, sz) && h == 0)
+                        return fail(LBM_ERR_GEOMETRY,
+                                    "fluid cell (" + std::to_string(x) + "," + std::to_string(y) +
+                                        "," + std::to_string(z) +
+                                        ") pulls from a ghost cell that no boundary covers");
+                    if (h != 0 && d->conds[h - 1].kind == LBM_BC_FREESLIP) {
+                        const int64_t cand[6][3] = {{x, sy, sz}, {sx, y, sz}, {sx, sy, z},
+                                                    {sx, y, z},  {x, sy, z},  {x, y, sz}};
+                        for (int j = 0; j < 6; ++j)
+                            if (!d->interior(cand[j][0], cand[j][1], cand[j][2]) &&
+                                d->handler[d->hidx(cand[j][0], cand[j][1], cand[j][2])] == 0)
+                                return fail(LBM_ERR_GEOMETRY,
+                                            "free-slip rule would read a ghost cell that no "
+                                            "boundary covers");
+                    }
+                }
+            }
+
+    // Distinct conditions (the rules depend only on kind and parameters).
+    std::vector<int> cond_map(d->conds.size(), -1);
+    std::vector<HostCond> uniq;
+    for (size_t i = 0; i < d->conds.size(); ++i) {
+        const HostCond& c = d->conds[i];
+        for (size_t j = 0; j < uniq.size(); ++j)
+            if (uniq[j].kind == c.kind && std::memcmp(uniq[j].vel, c.vel, sizeof(c.vel)) == 0 &&
+                std::memcmp(&uniq[j].rho, &c.rho, sizeof(double)) == 0) {
+                cond_map[i] = static_cast<int>(j);
+                break;
+            }
+        if (cond_map[i] < 0) {
+            cond_map[i] = static_cast<int>(uniq.size());
+            uniq.push_back(c);
+        }
+    }
+    if (static_cast<int>(uniq.size()) > kMaxConds)
+        return fail(LBM_ERR_UNSUPPORTED, "more than 252 distinct boundary conditions");
+    std::vector<Cond> table(uniq.size());
+    for (size_t j = 0; j < uniq.size(); ++j) {
+        Cond& c = table[j];
+        std::memset(&c, 0, sizeof(c));
+        c.kind = uniq[j].kind;
+        for (int k = 0; k < 3; ++k) c.vel[k] = uniq[j].vel[k];
+        c.rho = uniq[j].rho;
+        if (c.kind == LBM_BC_INFLOW) {
+            host_feq(lat, c.rho, c.vel, c.feq);
+            for (int q = 0; q < Q; ++q) {
+                const double cu = lat.c[q][0] * c.vel[0] + lat.c[q][1] * c.vel[1] + lat.c[q][2] * c.vel[2];
+                const double uu = c.vel[0] * c.vel[0] + c.vel[1] * c.vel[1] + c.vel[2] * c.vel[2];
+                c.geq[q] = lat.w[q] * ((c.rho - 1.0) + c.rho * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu));
+            }
+        }
+    }
+
+    // Cell kinds in the padded device layout, and the NEAR list.
+    std::vector<uint8_t> kind(static_cast<size_t>(L.plane), KIND_INERT);
+    std::vector<CellCoord> blist;
+    uint64_t nfluid = 0, nbulk = 0, nonbulk_interior = 0;
+    for (int z = 0; z < d->nz; ++z)
+        for (int y = 0; y < d->ny; ++y)
+            for (int x = 0; x < d->nx; ++x) {
+                const uint16_t h = d->handler[d->hidx(x, y, z)];
+                const int64_t e = cell_elem(L, x, y, z);
+                const bool in = d->interior(x, y, z);
+                if (h != 0) {
+                    kind[e] = static_cast<uint8_t>(KIND_COND0 + cond_map[h - 1]);
+                    if (in) ++nonbulk_interior;
+                    continue;
+                }
+                if (!in) continue;  // unassigned ghost: INERT
+                ++nfluid;
+                bool near = false;
+                for (int q = 0; q < Q && !near; ++q) {
+                    const int64_t sx = d->wrapc(x - lat.c[q][0], 0);
+                    const int64_t sy = d->wrapc(y - lat.c[q][1], 1);
+                    const int64_t sz = d->wrapc(z - lat.c[q][2], 2);
+                    near = d->handler[d->hidx(sx, sy, sz)] != 0;
+                }
+                if (near) {
+                    kind[e] = KIND_NEAR;
+                    blist.push_back(CellCoord{x, y, z, 0});
+                    ++nonbulk_interior;
+                } else {
+                    kind[e] = KIND_BULK;
+                    ++nbulk;
+                }
+            }
+
+    HIPCHK(hipMemcpyAsync(d->d_kind, kind.data(), kind.size(), hipMemcpyHostToDevice, d->stream));
+    d->nb = static_cast<int64_t>(blist.size());
+    if (d->nb) {
+        HIPCHK(hipMalloc(&d->d_blist, blist.size() * sizeof(CellCoord)));
+        HIPCHK(hipMemcpyAsync(d->d_blist, blist.data(), blist.size() * sizeof(CellCoord),
+                              hipMemcpyHostToDevice, d->stream));
+        d->device_bytes += blist.size() * sizeof(CellCoord);
+    }
+    if (!table.empty()) {
+        HIPCHK(hipMalloc(&d->d_conds, table.size() * sizeof(Cond)));
+        HIPCHK(hipMemcpyAsync(d->d_conds, table.data(), table.size() * sizeof(Cond),
+                              hipMemcpyHostToDevice, d->stream));
+    }
+    d->nconds_dev = static_cast<int>(table.size());
+    if (!d->h_alpha.empty()) {
+        const size_t cs = (d->mode == Mode::F64S) ? 8 : 4;
+        std::vector<char> a(static_cast<size_t>(L.plane) * cs, 0);
+        int64_t i = 0;
+        for (int z = 1; z <= L.zl; ++z)
+            for (int y = 1; y <= L.yl; ++y)
+                for (int x = 1; x <= L.xl; ++x, ++i) {
+                    const int64_t e = cell_elem(L, x, y, z);
+                    if (cs == 8)
+                        reinterpret_cast<double*>(a.data())[e] = d->h_alpha[i];
+                    else
+                        reinterpret_cast<float*>(a.data())[e] = static_cast<float>(d->h_alpha[i]);
+                }
+        HIPCHK(hipMalloc(&d->d_alpha, a.size()));
+        HIPCHK(hipMemcpyAsync(d->d_alpha, a.data(), a.size(), hipMemcpyHostToDevice, d->stream));
+        d->device_bytes += a.size();
+    }
+    d->nred = static_cast<int>(std::min<int64_t>(1024, (static_cast<int64_t>(L.xl) * L.yl * L.zl + 255) / 256));
+    HIPCHK(hipMalloc(&d->d_partial, sizeof(double) * 4 * d->nred));
+    HIPCHK(hipMalloc(&d->d_out4, sizeof(double) * 4));
+    HIPCHK(hipStreamSynchronize(d->stream));
+
+    d->n_fluid = nfluid;
+    d->plan.flags = nonbulk_interior > 0;
+    d->plan.brink = !d->h_alpha.empty();
+    d->plan.has_bulk = nbulk > 0;
+    if (int rc = make_plan(d)) return rc;
+    d->finalized = true;
+    return LBM_OK;
+}
+
+int lbm_set_tuning(lbm_domain* d, int vec, int block_x) {
+    if (!d) return fail(LBM_ERR_ARG, "null domain handle");
+    const int ov = d->tune_vec, ob = d->tune_bx;
+    d->tune_vec = vec;
+    d->tune_bx = block_x;
+    if (d->finalized) {
+        if (int rc = make_plan(d)) {
+            d->tune_vec = ov;
+            d->tune_bx = ob;
+            make_plan(d);
+            return rc;
+        }
+    }
+    return LBM_OK;
+}
+
+static int run_steps(lbm_domain* d, uint64_t nsteps, hipEvent_t* ev) {
+    for (uint64_t t = 0; t < nsteps; ++t) {
+        const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur ^ 1], d->materialised ? 1 : 0);
+        hipError_t e = d->ops->step(a, d->plan, d->stream, ev ? ev + 4 * t : nullptr);
+        if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e));
+        d->cur ^= 1;
+        d->materialised = false;
+    }
+    return LBM_OK;
+}
+
+int lbm_step(lbm_domain* d, uint64_t nsteps) {
+    if (int rc = check_handle(d)) return rc;
+    if (!d->finalized) return fail(LBM_ERR_STATE, "call lbm_finalize before lbm_step");
+    return run_steps(d, nsteps, nullptr);
+}
+
+int lbm_sync(lbm_domain* d) {
+    if (int rc = check_handle(d)) return rc;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return LBM_OK;
+}
+
+int lbm_time_steps(lbm_domain* d, uint64_t nsteps, double out[4]) {
+    if (int rc = check_handle(d)) return rc;
+    if (!d->finalized) return fail(LBM_ERR_STATE, "call lbm_finalize first");
+    if (!out || nsteps == 0 || nsteps > 100000) return fail(LBM_ERR_ARG, "nsteps in [1, 1e5]");
+    std::vector<hipEvent_t> ev(4 * nsteps, nullptr);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    int rc = run_steps(d, nsteps, ev.data());
+    if (rc == LBM_OK) {
+        HIPCHK(hipStreamSynchronize(d->stream));
+        float ms = 0;
+        double bulk = 0, near = 0;
+        for (uint64_t t = 0; t < nsteps; ++t) {
+            HIPCHK(hipEventElapsedTime(&ms, ev[4 * t + 0], ev[4 * t + 1]));
+            bulk += ms;
+            HIPCHK(hipEventElapsedTime(&ms, ev[4 * t + 2], ev[4 * t + 3]));
+            near += ms;
+        }
+        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[4 * nsteps - 1]));
+        out[0] = ms;
+        out[1] = bulk;
+        out[2] = near;
+        out[3] = static_cast<double>(nsteps);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int lbm_get_pdf(lbm_domain* d, double* out) {
+    if (int rc = check_handle(d)) return rc;
+    if (!out) return fail(LBM_ERR_ARG, "null output");
+    if (!d->finalized && !d->materialised) return fail(LBM_ERR_STATE, "inconsistent state");
+    if (d->finalized && !d->materialised) {
+        const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur], 0);
+        HIPCHK(d->ops->materialise(a, d->nx, d->ny, d->nz, d->stream));
+    }
+    const size_t bytes = static_cast<size_t>(d->ncell) * d->Q * sizeof(double);
+    double* dev = nullptr;
+    HIPCHK(hipMalloc(&dev, bytes));
+    hipError_t e = d->ops->get_pdf(d->L, d->buf[d->cur], dev, d->nx, d->ny, d->nz, d->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dev, bytes, hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("get_pdf: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_set_pdf(lbm_domain* d, const double* in) {
+    if (int rc = check_handle(d)) return rc;
+    if (!in) return fail(LBM_ERR_ARG, "null input");
+    const size_t bytes = static_cast<size_t>(d->ncell) * d->Q * sizeof(double);
+    double* dev = nullptr;
+    HIPCHK(hipMalloc(&dev, bytes));
+    hipError_t e = hipMemcpyAsync(dev, in, bytes, hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess) e = d->ops->set_pdf(d->L, d->buf[d->cur], dev, d->nx, d->ny, d->nz, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("set_pdf: ") + hipGetErrorString(e));
+    d->materialised = true;
+    return LBM_OK;
+}
+
+int lbm_init_equilibrium(lbm_domain* d, const double* rho, const double* u) {
+    if (int rc = check_handle(d)) return rc;
+    if (!d->finalized) return fail(LBM_ERR_STATE, "call lbm_finalize first (fluid cells are needed)");
+    if (!rho || !u) return fail(LBM_ERR_ARG, "null input");
+    const size_t n = static_cast<size_t>(d->L.xl) * d->L.yl * d->L.zl;
+    double* dev = nullptr;
+    HIPCHK(hipMalloc(&dev, 4 * n * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(dev, rho, n * sizeof(double), hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dev + n, u, 3 * n * sizeof(double), hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess) e = d->ops->init_eq(d->L, d->buf[d->cur], d->d_kind, dev, dev + n, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("init_equilibrium: ") + hipGetErrorString(e));
+    d->materialised = true;
+    return LBM_OK;
+}
+
+int lbm_moments(lbm_domain* d, double* rho, double* u) {
+    if (int rc = check_handle(d)) return rc;
+    const size_t n = static_cast<size_t>(d->L.xl) * d->L.yl * d->L.zl;
+    double* dev = nullptr;
+    HIPCHK(hipMalloc(&dev, 4 * n * sizeof(double)));
+    hipError_t e = d->ops->moments(d->L, d->buf[d->cur], dev, dev + n, d->stream);
+    if (e == hipSuccess && rho) e = hipMemcpyAsync(rho, dev, n * sizeof(double), hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess && u) e = hipMemcpyAsync(u, dev + n, 3 * n * sizeof(double), hipMemcpyDeviceToHost, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("moments: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_conservation(lbm_domain* d, double out[4]) {
+    if (int rc = check_handle(d)) return rc;
+    if (!d->finalized) return fail(LBM_ERR_STATE, "call lbm_finalize first");
+    if (!out) return fail(LBM_ERR_ARG, "null output");
+    HIPCHK(d->ops->conservation(d->L, d->buf[d->cur], d->d_kind, d->d_partial, d->nred, d->d_out4,
+                                d->stream));
+    HIPCHK(hipMemcpyAsync(out, d->d_out4, 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    return LBM_OK;
+}
+
+int lbm_get_info(lbm_domain* d, lbm_info* info) {
+    if (!d || !info) return fail(LBM_ERR_ARG, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->ncells_interior = static_cast<uint64_t>(d->L.xl) * d->L.yl * d->L.zl;
+    info->ncells_fluid = d->n_fluid;
+    info->ncells_near = static_cast<uint64_t>(d->nb);
+    info->bytes_per_value = d->esize;
+    info->field_bytes = d->field_bytes;
+    info->device_bytes = d->device_bytes;
+    info->vec = d->plan.V;
+    info->block_x = d->plan.BX;
+    info->block_y = d->plan.BY;
+    info->nconds = d->nconds_dev;
+    return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+// Collision operators and boundary rules, per cell, in registers.
+#pragma once
+
+#include "domain_view.hpp"
+#include "lattice.hpp"
+#include "storage.hpp"
+
+namespace lbm_amd {
+
+// ---------------------------------------------------------------------------
+// Reference BGK in fp64 (include/collision.hpp:7-70), operation for operation:
+//   rho = sum_q f_q (from 0, ascending q)
+//   m_d = sum_q f_q * c_qd (from 0, ascending q), u_d = m_d / rho
+//   feq_q = w_q * rho * (1 + cu/(C_S*C_S) + cu*cu/(2*C_S*C_S*C_S*C_S) - uu/(2*C_S*C_S))
+//   f_q -= (f_q - feq_q) / tau
+// Terms with c_qd = 0 add +-0 to a sum that starts at +0 and are left out:
+// x + (+-0) == x for every x this sum can hold, so the result bits are the
+// same. The translation unit that instantiates this is built without FMA
+// contraction.
+template <class L>
+__device__ __forceinline__ void moments_exact(const double (&f)[L::Q], double& rho, double (&u)[3]) {
+    rho = 0.0;
+    static_for<0, L::Q>([&](auto qc) { rho += f[decltype(qc)::value]; });
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (L::cx(q) > 0) m0 += f[q];
+        if constexpr (L::cx(q) < 0) m0 += -f[q];
+    });
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (L::cy(q) > 0) m1 += f[q];
+        if constexpr (L::cy(q) < 0) m1 += -f[q];
+    });
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (L::cz(q) > 0) m2 += f[q];
+        if constexpr (L::cz(q) < 0) m2 += -f[q];
+    });
+    u[0] = m0 / rho;
+    u[1] = m1 / rho;
+    u[2] = m2 / rho;
+}
+
+// Velocity with a caller-given density (Outflow/Pressure pass rho_ref,
+// include/boundary.hpp:239,304).
+template <class L>
+__device__ __forceinline__ void velocity_exact(const double (&f)[L::Q], double dens, double (&u)[3]) {
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (L::cx(q) > 0) m0 += f[q];
+        if constexpr (L::cx(q) < 0) m0 += -f[q];
+    });
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (L::cy(q) > 0) m1 += f[q];
+        if constexpr (L::cy(q) < 0) m1 += -f[q];
+    });
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (L::cz(q) > 0) m2 += f[q];
+        if constexpr (L::cz(q) < 0) m2 += -f[q];
+    });
+    u[0] = m0 / dens;
+    u[1] = m1 / dens;
+    u[2] = m2 / dens;
+}
+
+// One equilibrium value, reference expression tree.
+template <class L, int q>
+__device__ __forceinline__ double feq_exact(double rho, const double (&u)[3]) {
+    double cu = 0.0;
+    if constexpr (L::cx(q) > 0) cu += u[0];
+    if constexpr (L::cx(q) < 0) cu += -u[0];
+    if constexpr (L::cy(q) > 0) cu += u[1];
+    if constexpr (L::cy(q) < 0) cu += -u[1];
+    if constexpr (L::cz(q) > 0) cu += u[2];
+    if constexpr (L::cz(q) < 0) cu += -u[2];
+    double uu = 0.0;
+    uu += u[0] * u[0];
+    uu += u[1] * u[1];
+    uu += u[2] * u[2];
+    constexpr double cs = kCS;
+    return weight<L>(q) * rho *
+           (1 + cu / (cs * cs) + (cu) * (cu) / (2 * cs * cs * cs * cs) - uu / (2 * cs * cs));
+}
+
+template <class L>
+__device__ __forceinline__ void collide_exact(double (&f)[L::Q], double tau) {
+    double rho, u[3];
+    moments_exact<L>(f, rho, u);
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const double feq = feq_exact<L, q>(rho, u);
+        f[q] -= (f[q] - feq) / tau;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Shifted BGK: g_q = f_q - w_q, arithmetic in Calc (float or double),
+// 1/c_s^2 = 3 exactly. With a Brinkman field the equilibrium velocity is
+// (1 - alpha) * (u + tau * g) (porosity scaling of the equilibrium velocity
+// plus a velocity-shift body force).
+//   drho = sum g, rho = 1 + drho, u = sum c g / rho
+//   geq_q = w_q * (drho + rho * (3 cu + 4.5 cu^2 - 1.5 u.u))
+//   g_q  += omega * (geq_q - g_q)
+template <class L, class Calc, bool BRINK>
+__device__ __forceinline__ void collide_shifted(Calc (&g)[L::Q], Calc omega, Calc alpha,
+                                                Calc tgx, Calc tgy, Calc tgz) {
+    Calc drho = Calc(0), jx = Calc(0), jy = Calc(0), jz = Calc(0);
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        drho += g[q];
+        if constexpr (L::cx(q) > 0) jx += g[q];
+        if constexpr (L::cx(q) < 0) jx -= g[q];
+        if constexpr (L::cy(q) > 0) jy += g[q];
+        if constexpr (L::cy(q) < 0) jy -= g[q];
+        if constexpr (L::cz(q) > 0) jz += g[q];
+        if constexpr (L::cz(q) < 0) jz -= g[q];
+    });
+    const Calc rho = Calc(1) + drho;
+    const Calc inv_rho = Calc(1) / rho;
+    Calc ux = jx * inv_rho, uy = jy * inv_rho, uz = jz * inv_rho;
+    if constexpr (BRINK) {
+        const Calc s = Calc(1) - alpha;
+        ux = s * (ux + tgx);
+        uy = s * (uy + tgy);
+        uz = s * (uz + tgz);
+    }
+    const Calc base = drho - Calc(1.5) * rho * (ux * ux + uy * uy + uz * uz);
+    const Calc one_m_omega = Calc(1) - omega;
+    // Pairs (q, Q-1-q) share the even part of the equilibrium.
+    static_for<0, L::Q / 2>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int qb = L::Q - 1 - q;
+        constexpr Calc w = static_cast<Calc>(weight<L>(q));
+        Calc cu = Calc(0);
+        if constexpr (L::cx(q) > 0) cu += ux;
+        if constexpr (L::cx(q) < 0) cu -= ux;
+        if constexpr (L::cy(q) > 0) cu += uy;
+        if constexpr (L::cy(q) < 0) cu -= uy;
+        if constexpr (L::cz(q) > 0) cu += uz;
+        if constexpr (L::cz(q) < 0) cu -= uz;
+        const Calc even = w * (base + Calc(4.5) * rho * cu * cu);
+        const Calc odd = w * Calc(3) * rho * cu;
+        g[q] = one_m_omega * g[q] + omega * (even + odd);
+        g[qb] = one_m_omega * g[qb] + omega * (even - odd);
+    });
+    constexpr int r = L::rest;
+    constexpr Calc wr = static_cast<Calc>(weight<L>(r));
+    g[r] = one_m_omega * g[r] + omega * (wr * base);
+}
+
+}  // namespace lbm_amd

@@ -1,0 +1,369 @@
+// Fused pull-stream + collide kernels and the per-field kernels of one mode.
+//
+// One time step of the reference (src/main.cpp:50-52: stream, swap, collide)
+// is one pass here: every interior fluid cell x pulls f_q from x - c_q in the
+// source buffer (post-collision values of the previous step), collides in
+// registers and writes its Q values to the destination buffer.
+//
+// The reference's second collide pass (boundary rules on non-fluid cells,
+// include/domain.hpp:157-165) writes into a non-fluid cell b exactly those
+// directions q for which b + c_q is an interior fluid cell, and the only
+// reader of that value is the pull of that fluid cell in the next step. So a
+// fluid cell x that pulls direction q from a non-fluid b = x - c_q evaluates
+// b's rule for (b, q) itself, from its own post-collision values (all rules
+// read only the neighbour b + c_q = x, except FreeSlip, which reads fluid
+// cells next to b). Right after initialisation or after the host wrote the
+// field, non-fluid cells hold values that the next pull reads as stored
+// (the first step of the reference pulls the initial weights from every
+// boundary cell); the `materialised` flag selects that.
+//
+// Two kernels per step:
+//   k_bulk - every interior cell of kind BULK (all Q sources fluid), V cells
+//            per thread along x, one plane-contiguous load per direction.
+//   k_near - the listed cells of kind NEAR (fluid with a non-fluid source),
+//            one thread per cell, evaluates boundary rules.
+#pragma once
+
+#include "collide.hpp"
+#include "domain_view.hpp"
+#include "lattice.hpp"
+#include "storage.hpp"
+
+namespace lbm_amd {
+
+template <class T, int V>
+struct alignas(sizeof(T) * V) VecT {
+    T v[V];
+};
+
+__device__ __forceinline__ int wrap1(int v, int l) { return v == 0 ? l : (v == l + 1 ? 1 : v); }
+
+__device__ __forceinline__ int wrap_if(int v, int l, int per) {
+    return per ? wrap1(v, l) : v;
+}
+
+// ---------------------------------------------------------------------------
+// Bulk kernel. Block (BX, BY) threads covers BX*V cells of BY rows at one z.
+// Grid is flattened: blockIdx.x = bx + nbx * (by + nby * (z - 1)).
+//
+// Source rows are wrapped unconditionally: along a periodic axis that is the
+// periodic image, and along a walled axis any cell next to the wall pulls
+// from a ghost cell, is NEAR and is masked out before its values are used.
+template <class L, Mode M, int V, bool FLAGS, bool BRINK>
+__global__ void __launch_bounds__(256) k_bulk(StepArgs a, int nbx, int nby) {
+    using IO = Io<M, L>;
+    using T = typename IO::T;
+    using Calc = typename IO::Calc;
+    constexpr int Q = L::Q;
+    const FieldLayout& Ly = a.L;
+
+    const int b = blockIdx.x;
+    const int bx = b % nbx;
+    const int t = b / nbx;
+    const int by = t % nby;
+    const int z = 1 + t / nby;
+    const int xv = bx * blockDim.x + threadIdx.x;
+    const int y = 1 + by * blockDim.y + threadIdx.y;
+    const int x0 = 1 + xv * V;
+    if (x0 > Ly.xl || y > Ly.yl) return;
+
+    const int64_t e0 = cell_elem(Ly, x0, y, z);
+    bool act[V];
+    bool any = false;
+    if constexpr (FLAGS) {
+        const VecT<uint8_t, V> kv = *reinterpret_cast<const VecT<uint8_t, V>*>(a.kind + e0);
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            act[i] = (kv.v[i] == KIND_BULK) && (x0 + i <= Ly.xl);
+            any |= act[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            act[i] = (x0 + i <= Ly.xl);
+            any |= act[i];
+        }
+    }
+    if (!any) return;
+
+    const int ym = wrap1(y - 1, Ly.yl), yp = wrap1(y + 1, Ly.yl);
+    const int zm = wrap1(z - 1, Ly.zl), zp = wrap1(z + 1, Ly.zl);
+    const int xm = wrap1(x0 - 1, Ly.xl);  // source of lane 0 for cx = +1
+    const int xpv = wrap1(x0 + V, Ly.xl); // source of lane V-1 for cx = -1
+    const bool partial = (x0 + V - 1 > Ly.xl);
+
+    const T* __restrict__ src = static_cast<const T*>(a.src);
+    T* __restrict__ dst = static_cast<T*>(a.dst);
+
+    Calc f[V][Q];
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
+        const int sy = cy > 0 ? ym : (cy < 0 ? yp : y);
+        const int sz = cz > 0 ? zm : (cz < 0 ? zp : z);
+        const T* __restrict__ row = src + q * Ly.plane + Ly.off + Ly.px * sy + Ly.pxy * sz;
+        T v[V];
+        if constexpr (V == 1) {
+            v[0] = row[cx == 0 ? x0 : (cx > 0 ? xm : xpv)];
+        } else {
+            const VecT<T, V> A = *reinterpret_cast<const VecT<T, V>*>(row + x0);
+            if constexpr (cx == 0) {
+#pragma unroll
+                for (int i = 0; i < V; ++i) v[i] = A.v[i];
+            } else if constexpr (cx > 0) {
+                v[0] = row[xm];
+#pragma unroll
+                for (int i = 1; i < V; ++i) v[i] = A.v[i - 1];
+            } else {
+#pragma unroll
+                for (int i = 0; i < V - 1; ++i) v[i] = A.v[i + 1];
+                v[V - 1] = row[xpv];
+                // last, partially filled vector of a periodic row: lane of x = xl
+                // takes x = 1 (on a walled row that lane is NEAR and masked)
+                if (partial) {
+#pragma unroll
+                    for (int i = 0; i < V - 1; ++i)
+                        if (x0 + i == Ly.xl) v[i] = row[1];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < V; ++i) f[i][q] = IO::template to_calc<q>(v[i]);
+    });
+
+    Calc alpha[V];
+    if constexpr (BRINK) {
+        const Calc* al = static_cast<const Calc*>(a.alpha);
+#pragma unroll
+        for (int i = 0; i < V; ++i) alpha[i] = act[i] ? al[e0 + i] : Calc(0);
+    }
+
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        if constexpr (M == Mode::EXACT64) {
+            collide_exact<L>(f[i], a.tau);
+        } else {
+            collide_shifted<L, Calc, BRINK>(f[i], static_cast<Calc>(a.omega),
+                                            BRINK ? alpha[i] : Calc(0),
+                                            static_cast<Calc>(a.tau * a.g[0]),
+                                            static_cast<Calc>(a.tau * a.g[1]),
+                                            static_cast<Calc>(a.tau * a.g[2]));
+        }
+    }
+
+    bool all = true;
+#pragma unroll
+    for (int i = 0; i < V; ++i) all &= act[i];
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        T* __restrict__ out = dst + q * Ly.plane + e0;
+        if (all) {
+            VecT<T, V> o;
+#pragma unroll
+            for (int i = 0; i < V; ++i) o.v[i] = IO::template from_calc<q>(f[i][q]);
+            *reinterpret_cast<VecT<T, V>*>(out) = o;
+        } else {
+#pragma unroll
+            for (int i = 0; i < V; ++i)
+                if (act[i]) out[i] = IO::template from_calc<q>(f[i][q]);
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Boundary rules, evaluated by the fluid cell x for a non-fluid source b.
+// own[] holds x's post-collision values (the values the reference's rule
+// reads from its fluid neighbour), in the Calc representation of the mode.
+
+// Reference-order rule values (EXACT64), include/boundary.hpp.
+template <class L>
+struct RulesExact {
+    static constexpr int Q = L::Q;
+    // MovingWall, boundary.hpp:150-161
+    template <int q>
+    __device__ static double moving_wall(const double (&own)[Q], const Cond& cd) {
+        double density = 0;
+        static_for<0, Q>([&](auto kc) { density += own[decltype(kc)::value]; });
+        double c_dot_u = 0;
+        c_dot_u += static_cast<double>(L::cx(q)) * cd.vel[0];
+        c_dot_u += static_cast<double>(L::cy(q)) * cd.vel[1];
+        c_dot_u += static_cast<double>(L::cz(q)) * cd.vel[2];
+        const double finv = own[Q - 1 - q];
+        constexpr double cs = kCS;
+        return finv + 2.0 * weight<L>(q) * density * c_dot_u / (cs * cs);
+    }
+    // Outflow boundary.hpp:235-243 and Pressure boundary.hpp:301-307
+    template <int q>
+    __device__ static double outflow(const double (&own)[Q], const Cond& cd) {
+        double u[3];
+        velocity_exact<L>(own, cd.rho, u);
+        const double fq = feq_exact<L, q>(cd.rho, u);
+        const double fb = feq_exact<L, Q - 1 - q>(cd.rho, u);
+        return fq + fb - own[Q - 1 - q];
+    }
+};
+
+// Shifted-value rule values (g = f - w), same rules.
+template <class L, class Calc>
+struct RulesShifted {
+    static constexpr int Q = L::Q;
+    template <int q>
+    __device__ static Calc moving_wall(const Calc (&own)[Q], const Cond& cd) {
+        Calc drho = Calc(0);
+        static_for<0, Q>([&](auto kc) { drho += own[decltype(kc)::value]; });
+        const Calc rho = Calc(1) + drho;
+        const Calc cu = static_cast<Calc>(L::cx(q) * cd.vel[0] + L::cy(q) * cd.vel[1] +
+                                          L::cz(q) * cd.vel[2]);
+        return own[Q - 1 - q] + static_cast<Calc>(6.0 * weight<L>(q)) * rho * cu;
+    }
+    template <int q>
+    __device__ static Calc outflow(const Calc (&own)[Q], const Cond& cd) {
+        Calc jx = Calc(0), jy = Calc(0), jz = Calc(0);
+        static_for<0, Q>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if constexpr (L::cx(k) > 0) jx += own[k];
+            if constexpr (L::cx(k) < 0) jx -= own[k];
+            if constexpr (L::cy(k) > 0) jy += own[k];
+            if constexpr (L::cy(k) < 0) jy -= own[k];
+            if constexpr (L::cz(k) > 0) jz += own[k];
+            if constexpr (L::cz(k) < 0) jz -= own[k];
+        });
+        const Calc rr = static_cast<Calc>(cd.rho);
+        const Calc ir = Calc(1) / rr;
+        const Calc ux = jx * ir, uy = jy * ir, uz = jz * ir;
+        const Calc cu = Calc(L::cx(q)) * ux + Calc(L::cy(q)) * uy + Calc(L::cz(q)) * uz;
+        const Calc uu = ux * ux + uy * uy + uz * uz;
+        // (feq_q - w) + (feq_qbar - w): the odd part cancels
+        const Calc even = (rr - Calc(1)) + rr * (Calc(4.5) * cu * cu - Calc(1.5) * uu);
+        return static_cast<Calc>(2.0 * weight<L>(q)) * even - own[Q - 1 - q];
+    }
+};
+
+// Reflected direction for the FreeSlip chain (include/model.h velocity_index).
+template <class L>
+__device__ __forceinline__ int dir_index(int cx, int cy, int cz) {
+    int r = 0;
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if (L::cx(q) == cx && L::cy(q) == cy && L::cz(q) == cz) r = q;
+    });
+    return r;
+}
+
+// Value that fluid cell x = (x,y,z) pulls in direction q from the non-fluid
+// cell b = (sx,sy,sz) (element se) with condition cd: the value the reference's
+// rule writes into b[q] in collide pass 2 (include/boundary.hpp). Sets stale
+// when the rule writes nothing (FreeSlip with no fluid candidate); the value
+// returned is then b's stored value, which is what the reference reads too.
+template <class L, Mode M, int q>
+__device__ __forceinline__ typename Io<M, L>::Calc pull_nonfluid(
+    const StepArgs& a, const Cond& cd, const typename Io<M, L>::Calc (&own)[L::Q], int x, int y,
+    int z, int sx, int sy, int sz, int64_t se, bool& stale) {
+    using IO = Io<M, L>;
+    using T = typename IO::T;
+    using Calc = typename IO::Calc;
+    constexpr int Q = L::Q;
+    constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
+    const FieldLayout& Ly = a.L;
+    const T* __restrict__ src = static_cast<const T*>(a.src);
+    stale = false;
+    switch (cd.kind) {
+        case BC_NOSLIP:  // boundary.hpp:121-124
+            return own[Q - 1 - q];
+        case BC_MOVINGWALL:
+            if constexpr (M == Mode::EXACT64)
+                return RulesExact<L>::template moving_wall<q>(own, cd);
+            else
+                return RulesShifted<L, Calc>::template moving_wall<q>(own, cd);
+        case BC_INFLOW:  // boundary.hpp:271-274, feq precomputed on the host
+            return (M == Mode::EXACT64) ? static_cast<Calc>(cd.feq[q]) : static_cast<Calc>(cd.geq[q]);
+        case BC_OUTFLOW:
+        case BC_PRESSURE:
+            if constexpr (M == Mode::EXACT64)
+                return RulesExact<L>::template outflow<q>(own, cd);
+            else
+                return RulesShifted<L, Calc>::template outflow<q>(own, cd);
+        case BC_FREESLIP: {  // boundary.hpp:186-208 with b = (sx,sy,sz), b + c_q = x
+            const int cand[6][3] = {{x, sy, sz}, {sx, y, sz}, {sx, sy, z},
+                                    {sx, y, z},  {x, sy, z},  {x, y, sz}};
+            const int refl[6][3] = {{-cx, cy, cz},  {cx, -cy, cz},  {cx, cy, -cz},
+                                    {cx, -cy, -cz}, {-cx, cy, -cz}, {-cx, -cy, cz}};
+            for (int j = 0; j < 6; ++j) {
+                const int64_t ce = cell_elem(Ly, cand[j][0], cand[j][1], cand[j][2]);
+                if (a.kind[ce] < KIND_COND0) {
+                    const int rq = dir_index<L>(refl[j][0], refl[j][1], refl[j][2]);
+                    return IO::to_calc_rt(src[rq * Ly.plane + ce], rq);
+                }
+            }
+            stale = true;
+            return IO::template to_calc<q>(src[q * Ly.plane + se]);
+        }
+        default:
+            stale = true;
+            return IO::template to_calc<q>(src[q * Ly.plane + se]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Near-boundary kernel: one thread per listed NEAR cell.
+template <class L, Mode M>
+__global__ void __launch_bounds__(128) k_near(StepArgs a) {
+    using IO = Io<M, L>;
+    using T = typename IO::T;
+    using Calc = typename IO::Calc;
+    constexpr int Q = L::Q;
+    const FieldLayout& Ly = a.L;
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= a.nb) return;
+    const CellCoord c = a.blist[i];
+    const int x = c.x, y = c.y, z = c.z;
+    const int64_t e = cell_elem(Ly, x, y, z);
+    const T* __restrict__ src = static_cast<const T*>(a.src);
+    T* __restrict__ dst = static_cast<T*>(a.dst);
+
+    Calc own[Q];
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        own[q] = IO::template to_calc<q>(src[q * Ly.plane + e]);
+    });
+
+    Calc f[Q];
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
+        const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
+        const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
+        const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
+        const int64_t se = cell_elem(Ly, sx, sy, sz);
+        const uint8_t k = a.kind[se];
+        if (k < KIND_COND0 || a.materialised) {
+            f[q] = IO::template to_calc<q>(src[q * Ly.plane + se]);
+        } else {
+            bool stale;
+            f[q] = pull_nonfluid<L, M, q>(a, a.conds[k - KIND_COND0], own, x, y, z, sx, sy, sz, se,
+                                          stale);
+        }
+    });
+
+    if constexpr (M == Mode::EXACT64) {
+        collide_exact<L>(f, a.tau);
+    } else {
+        Calc alpha = Calc(0);
+        if (a.alpha) alpha = static_cast<const Calc*>(a.alpha)[e];
+        if (a.alpha) {
+            collide_shifted<L, Calc, true>(f, static_cast<Calc>(a.omega), alpha,
+                                           static_cast<Calc>(a.tau * a.g[0]),
+                                           static_cast<Calc>(a.tau * a.g[1]),
+                                           static_cast<Calc>(a.tau * a.g[2]));
+        } else {
+            collide_shifted<L, Calc, false>(f, static_cast<Calc>(a.omega), Calc(0), Calc(0),
+                                            Calc(0), Calc(0));
+        }
+    }
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        dst[q * Ly.plane + e] = IO::template from_calc<q>(f[q]);
+    });
+}
+
+}  // namespace lbm_amd

@@ -1,0 +1,25 @@
+// Shifted-storage kernels (F32, F16, BF16 with fp32 arithmetic; F64S with
+// fp64 arithmetic) for D3Q19.
+#include "ops_impl.hpp"
+
+namespace lbm_amd {
+
+namespace {
+using Lat = D3Q19;
+const ModeOps kF32 = OpsImpl<Lat, Mode::F32>::make<1, 2, 4>(4);
+const ModeOps kF16 = OpsImpl<Lat, Mode::F16>::make<2, 4, 8>(8);
+const ModeOps kBF16 = OpsImpl<Lat, Mode::BF16>::make<2, 4, 8>(8);
+const ModeOps kF64S = OpsImpl<Lat, Mode::F64S>::make<1, 2>(1);
+}  // namespace
+
+const ModeOps* ops_fast_q19(Mode m) {
+    switch (m) {
+        case Mode::F32: return &kF32;
+        case Mode::F16: return &kF16;
+        case Mode::BF16: return &kBF16;
+        case Mode::F64S: return &kF64S;
+        default: return nullptr;
+    }
+}
+
+}  // namespace lbm_amd

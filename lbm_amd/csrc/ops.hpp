@@ -1,0 +1,66 @@
+// Host-side launch table of one (lattice, storage mode) pair. Each mode
+// translation unit fills one ModeOps per lattice; the C-ABI picks the table
+// when a domain is created and calls through it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "domain_view.hpp"
+#include "storage.hpp"
+
+namespace lbm_amd {
+
+struct GhostDims;
+
+struct StepPlan {
+    int V;          // cells per thread along x in k_bulk
+    int BX, BY;     // k_bulk block shape (BX * BY == 256)
+    int nbx, nby;   // k_bulk blocks along x-vectors and rows
+    int64_t nblocks_bulk;
+    bool flags;     // domain has non-bulk interior cells: read the kind array
+    bool brink;     // Brinkman field present
+    bool has_bulk;  // at least one BULK cell
+};
+
+struct ModeOps {
+    Mode mode;
+    int q;
+    int default_vec;
+    // Supported values of StepPlan::V, 0-terminated.
+    int vecs[4];
+    // bulk + near for one step (src/dst in args). ev (may be null) receives
+    // records before/after the bulk and near launches: ev[0..3].
+    hipError_t (*step)(const StepArgs& a, const StepPlan& p, hipStream_t s, hipEvent_t* ev);
+    hipError_t (*get_pdf)(const FieldLayout& Ly, const void* src, double* out, int nx, int ny,
+                          int nz, hipStream_t s);
+    hipError_t (*set_pdf)(const FieldLayout& Ly, void* dst, const double* in, int nx, int ny,
+                          int nz, hipStream_t s);
+    hipError_t (*moments)(const FieldLayout& Ly, const void* src, double* rho, double* u,
+                          hipStream_t s);
+    // partial must hold 4 * nblocks doubles; out 4 doubles.
+    hipError_t (*conservation)(const FieldLayout& Ly, const void* src, const uint8_t* kind,
+                               double* partial, int nblocks, double* out, hipStream_t s);
+    hipError_t (*init_eq)(const FieldLayout& Ly, void* dst, const uint8_t* kind,
+                          const double* rho, const double* u, hipStream_t s);
+    hipError_t (*materialise)(const StepArgs& a, int nx, int ny, int nz, hipStream_t s);
+};
+
+// Defined in the mode translation units.
+const ModeOps* ops_exact(int q);
+const ModeOps* ops_fast_q15(Mode m);
+const ModeOps* ops_fast_q19(Mode m);
+const ModeOps* ops_fast_q27(Mode m);
+
+inline const ModeOps* mode_ops(int q, Mode m) {
+    if (m == Mode::EXACT64) return ops_exact(q);
+    switch (q) {
+        case 15: return ops_fast_q15(m);
+        case 19: return ops_fast_q19(m);
+        case 27: return ops_fast_q27(m);
+        default: return nullptr;
+    }
+}
+
+}  // namespace lbm_amd

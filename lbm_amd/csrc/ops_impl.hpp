@@ -1,0 +1,122 @@
+// Launchers behind ModeOps, instantiated per (lattice, mode) by the mode
+// translation units.
+#pragma once
+
+#include "kernels.hpp"
+#include "kernels_fields.hpp"
+#include "ops.hpp"
+
+namespace lbm_amd {
+
+template <class L, Mode M>
+struct OpsImpl {
+    template <int V, bool FLAGS, bool BRINK>
+    static void launch_bulk(const StepArgs& a, const StepPlan& p, hipStream_t s) {
+        dim3 block(p.BX, p.BY, 1);
+        hipLaunchKernelGGL((k_bulk<L, M, V, FLAGS, BRINK>), dim3(static_cast<uint32_t>(p.nblocks_bulk)),
+                           block, 0, s, a, p.nbx, p.nby);
+    }
+
+    template <int V>
+    static bool try_bulk(const StepArgs& a, const StepPlan& p, hipStream_t s) {
+        if (p.V != V) return false;
+        if (p.brink) {
+            if (p.flags)
+                launch_bulk<V, true, true>(a, p, s);
+            else
+                launch_bulk<V, false, true>(a, p, s);
+        } else {
+            if (p.flags)
+                launch_bulk<V, true, false>(a, p, s);
+            else
+                launch_bulk<V, false, false>(a, p, s);
+        }
+        return true;
+    }
+
+    template <int... Vs>
+    static hipError_t step_impl(const StepArgs& a, const StepPlan& p, hipStream_t s,
+                                hipEvent_t* ev) {
+        if (ev) (void)hipEventRecord(ev[0], s);
+        if (p.has_bulk) {
+            bool done = (try_bulk<Vs>(a, p, s) || ...);
+            if (!done) return hipErrorInvalidValue;
+        }
+        if (ev) (void)hipEventRecord(ev[1], s);
+        if (ev) (void)hipEventRecord(ev[2], s);
+        if (a.nb > 0) {
+            const int bs = 128;
+            const int64_t nblk = (a.nb + bs - 1) / bs;
+            hipLaunchKernelGGL((k_near<L, M>), dim3(static_cast<uint32_t>(nblk)), dim3(bs), 0, s, a);
+        }
+        if (ev) (void)hipEventRecord(ev[3], s);
+        return hipGetLastError();
+    }
+
+    static hipError_t get_pdf(const FieldLayout& Ly, const void* src, double* out, int nx, int ny,
+                              int nz, hipStream_t s) {
+        const int64_t n = static_cast<int64_t>(nx) * ny * nz;
+        const int bs = 256;
+        hipLaunchKernelGGL((k_get_pdf<L, M>), dim3(static_cast<uint32_t>((n + bs - 1) / bs)),
+                           dim3(bs), 0, s, Ly, src, out, GhostDims{nx, ny, nz});
+        return hipGetLastError();
+    }
+    static hipError_t set_pdf(const FieldLayout& Ly, void* dst, const double* in, int nx, int ny,
+                              int nz, hipStream_t s) {
+        const int64_t n = static_cast<int64_t>(nx) * ny * nz;
+        const int bs = 256;
+        hipLaunchKernelGGL((k_set_pdf<L, M>), dim3(static_cast<uint32_t>((n + bs - 1) / bs)),
+                           dim3(bs), 0, s, Ly, dst, in, GhostDims{nx, ny, nz});
+        return hipGetLastError();
+    }
+    static hipError_t moments(const FieldLayout& Ly, const void* src, double* rho, double* u,
+                              hipStream_t s) {
+        const int64_t n = static_cast<int64_t>(Ly.xl) * Ly.yl * Ly.zl;
+        const int bs = 256;
+        hipLaunchKernelGGL((k_moments<L, M>), dim3(static_cast<uint32_t>((n + bs - 1) / bs)),
+                           dim3(bs), 0, s, Ly, src, rho, u);
+        return hipGetLastError();
+    }
+    static hipError_t conservation(const FieldLayout& Ly, const void* src, const uint8_t* kind,
+                                   double* partial, int nblocks, double* out, hipStream_t s) {
+        hipLaunchKernelGGL((k_conservation<L, M>), dim3(nblocks), dim3(256), 0, s, Ly, src, kind,
+                           partial);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partial, nblocks, out);
+        return hipGetLastError();
+    }
+    static hipError_t init_eq(const FieldLayout& Ly, void* dst, const uint8_t* kind,
+                              const double* rho, const double* u, hipStream_t s) {
+        const int64_t n = static_cast<int64_t>(Ly.xl) * Ly.yl * Ly.zl;
+        const int bs = 256;
+        hipLaunchKernelGGL((k_init_eq<L, M>), dim3(static_cast<uint32_t>((n + bs - 1) / bs)),
+                           dim3(bs), 0, s, Ly, dst, kind, rho, u);
+        return hipGetLastError();
+    }
+    static hipError_t materialise(const StepArgs& a, int nx, int ny, int nz, hipStream_t s) {
+        const int64_t n = static_cast<int64_t>(nx) * ny * nz;
+        const int bs = 128;
+        hipLaunchKernelGGL((k_materialise<L, M>), dim3(static_cast<uint32_t>((n + bs - 1) / bs)),
+                           dim3(bs), 0, s, a, GhostDims{nx, ny, nz});
+        return hipGetLastError();
+    }
+
+    template <int V0, int... Vs>
+    static ModeOps make(int default_vec) {
+        ModeOps o{};
+        o.mode = M;
+        o.q = L::Q;
+        o.default_vec = default_vec;
+        const int v[] = {V0, Vs...};
+        for (int i = 0; i < 4; ++i) o.vecs[i] = i < static_cast<int>(sizeof(v) / sizeof(int)) ? v[i] : 0;
+        o.step = &step_impl<V0, Vs...>;
+        o.get_pdf = &get_pdf;
+        o.set_pdf = &set_pdf;
+        o.moments = &moments;
+        o.conservation = &conservation;
+        o.init_eq = &init_eq;
+        o.materialise = &materialise;
+        return o;
+    }
+};
+
+}  // namespace lbm_amd
