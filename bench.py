@@ -1,0 +1,172 @@
+"""Benchmark of the collide-stream time step on one MI355X.
+
+Workload: periodic 256^3 D3Q19 box, BGK tau = 0.8, 2-D Taylor-Green vortex
+(u0 = 0.02, k = 2 pi / 256) plus a seeded density perturbation 1e-4 N(0,1),
+shifted distributions stored as fp32 (the headline) and as fp16 / bf16, all
+with fp32 arithmetic. A step is one fused pull-stream + collide pass over all
+16.8 M cells; inputs are resident in HBM when timing starts.
+
+Prints one JSON line: MLUPS of the fp32 mode (interior cells x steps / s),
+the roofline of the dominant kernel (algorithmic bytes 2*Q*sizeof(storage)
+per cell over its HIP-event-timed average launch), the fp16/bf16 modes and a
+CPU baseline (the fp64 CPU oracle, OpenMP on the host cores, bounded sample).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--modes f32,f16,bf16]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+N = 256
+Q = 19
+TAU = 0.8
+BYTES = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}
+
+
+def tgv(n: int, seed: int = 12345):
+    k = 2 * np.pi / n
+    x = np.arange(n)[None, None, :]
+    y = np.arange(n)[None, :, None]
+    u = np.zeros((n, n, n, 3))
+    u[..., 0] = 0.02 * np.sin(k * x) * np.cos(k * y)
+    u[..., 1] = -0.02 * np.cos(k * x) * np.sin(k * y)
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 1e-4 * rng.standard_normal((n, n, n))
+    return rho, u
+
+
+def traffic_from_profiles(mode: str):
+    """HBM bytes per bulk-kernel launch measured by rocprofv3 --pmc for this
+    workload (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel
+    calibration), from the committed summary; None when absent."""
+    p = ROOT / "profiles" / f"pmc_bulk_d3q19_256_{mode}.txt"
+    if not p.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text(), flags=re.M)
+    return float(m.group(1)) if m else None
+
+
+def run_mode(capi, mode: str, steps: int, warmup: int, rho, u):
+    d = capi.Domain(N, N, N, TAU, q=Q, storage=mode, periodic=(True, True, True))
+    d.finalize()
+    d.init_equilibrium(rho, u)
+    d.step(warmup)
+    d.sync()
+    t0 = time.perf_counter()
+    ev = d.time_steps(steps)  # HIP events around every kernel launch, on the domain's stream
+    d.sync()
+    wall = time.perf_counter() - t0
+    cells = N * N * N
+    mass = d.conservation()
+    info = d.info()
+    d.close()
+    kernel_ms = ev["bulk_ms"] / steps
+    alg_bytes = 2 * Q * BYTES[mode] * cells
+    return {
+        "mlups": cells * steps / wall / 1e6,
+        "ms_per_step": wall * 1e3 / steps,
+        "event_ms_per_step": ev["total_ms"] / steps,
+        "kernel_ms": kernel_ms,
+        "alg_bytes": alg_bytes,
+        "achieved_gbs": alg_bytes / (kernel_ms * 1e-3) / 1e9,
+        "mass": float(mass[0]),
+        "vec": info["vec"],
+        "block": [info["block_x"], info["block_y"]],
+    }
+
+
+def cpu_baseline(rho, u, steps: int = 4):
+    """The fp64 CPU oracle (the reference algorithm in its AoS layout) on the
+    same 256^3 D3Q19 periodic workload, OpenMP over the host cores."""
+    from oracle import ora
+    o = ora.OracleDomain(N, N, N, TAU, q=Q, periodic=(True, True, True))
+    o.finalize()
+    o.init_equilibrium(rho, u)
+    o.step(1)  # touch pages
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    cores = ora.num_threads()
+    return {"value": N ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"fp64 CPU oracle, D3Q19 256^3 periodic TGV, {steps} steps after 1 untimed, "
+                      f"{cores} OpenMP threads ({os.cpu_count()} logical CPUs visible), "
+                      f"{dt:.1f} s"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--modes", default="f32,f16,bf16")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="short copy-bandwidth run (3 copies) for rocprofv3 --pmc passes")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs one domain on one GPU; --gpus {a.gpus} is not supported "
+              "(multi-GPU slab decomposition is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    from lbm_amd import capi
+    capi.load()
+    # plain 1 GiB device copy: achievable-bandwidth reference, and the kernel of
+    # known size that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters
+    copy = capi.copy_bandwidth(1 << 30, reps=3 if a.calibrate else 10)
+    rho, u = tgv(N)
+    modes = [m for m in a.modes.split(",") if m]
+    res = {m: run_mode(capi, m, a.steps, a.warmup, rho, u) for m in modes}
+    head = res.get("f32") or res[modes[0]]
+    head_mode = "f32" if "f32" in res else modes[0]
+    traffic = traffic_from_profiles(head_mode)
+    line = {
+        "metric": "MLUPS + %HBM roofline, D3Q19 256^3 fp32 vs fp16-store",
+        "value": round(head["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(head["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Taylor-Green vortex + density noise, seed 12345)",
+        "config": {"workload": "D3Q19 periodic 256^3 TGV, BGK tau=0.8, shifted f-w stored fp32",
+                   "lattice": "D3Q19", "shape": [N, N, N], "tau": TAU, "storage": head_mode,
+                   "vec": head["vec"], "block": head["block"]},
+        "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "k_bulk", "kernel_ms": round(head["kernel_ms"], 5),
+                     "alg_bytes_per_launch": head["alg_bytes"],
+                     "copy_gbs": round(copy["gbs"], 1),
+                     "frac_of_copy": round(head["achieved_gbs"] / copy["gbs"], 4)},
+        "modes": {m: {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
+                      "achieved_gbs": round(r["achieved_gbs"], 1),
+                      "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                      "bytes_per_cell": 2 * Q * BYTES[m]} for m, r in res.items()},
+    }
+    if "f16" in res and "f32" in res:
+        line["f16_over_f32"] = round(res["f16"]["mlups"] / res["f32"]["mlups"], 3)
+    line["cpu_baseline"] = None if a.no_cpu else cpu_baseline(rho, u, a.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

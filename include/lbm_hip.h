@@ -153,6 +153,13 @@ int lbm_conservation(lbm_domain* d, double out[4]);
  * out[2] = ms in the near-boundary kernel, out[3] = number of steps. */
 int lbm_time_steps(lbm_domain* d, uint64_t nsteps, double out[4]);
 
+/* Extension: bandwidth of a plain device copy (16-byte vector loads and
+ * stores) of `bytes` bytes on `device`, repeated `reps` times after one
+ * warm-up copy. out[0] = ms per copy, out[1] = GB/s counting read + write.
+ * Used to calibrate HBM traffic counters and as the achievable-bandwidth
+ * reference next to the roofline. */
+int lbm_copy_bandwidth(int device, uint64_t bytes, int reps, double out[2]);
+
 /* Layout and size information. */
 typedef struct lbm_info {
     uint64_t ncells_interior;  /* xl*yl*zl */

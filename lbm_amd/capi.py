@@ -14,7 +14,9 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "liblbm_hip.so"
+import os
+
+LIB_PATH = Path(os.environ.get("LBM_HIP_LIB", Path(__file__).resolve().parent / "liblbm_hip.so"))
 
 STORAGE = {"f64": 0, "f32": 1, "f16": 2, "bf16": 3, "f64s": 4}
 STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "f64s": 8}
@@ -66,6 +68,7 @@ SIGNATURES = [
     ("lbm_moments", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
     ("lbm_conservation", c_int, [c_void_p, POINTER(c_double)]),
     ("lbm_time_steps", c_int, [c_void_p, c_uint64, POINTER(c_double)]),
+    ("lbm_copy_bandwidth", c_int, [c_int, c_uint64, c_int, POINTER(c_double)]),
     ("lbm_get_info", c_int, [c_void_p, POINTER(Info)]),
     ("lbm_set_tuning", c_int, [c_void_p, c_int, c_int]),
 ]
@@ -101,6 +104,13 @@ def lattice(q: int):
     _check(load().lbm_lattice(q, c.ctypes.data_as(POINTER(c_int)),
                               w.ctypes.data_as(POINTER(c_double)), byref(cs)))
     return c.astype(np.int64), w, cs.value
+
+
+def copy_bandwidth(nbytes: int, reps: int = 10, device: int = 0) -> dict:
+    """Plain device copy of nbytes (lbm_copy_bandwidth): ms per copy and GB/s."""
+    out = np.zeros(2)
+    _check(load().lbm_copy_bandwidth(device, nbytes, reps, out.ctypes.data_as(POINTER(c_double))))
+    return {"ms": float(out[0]), "gbs": float(out[1]), "bytes": int(nbytes // 16 * 16)}
 
 
 def _dptr(a: np.ndarray):

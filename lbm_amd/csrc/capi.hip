@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -99,6 +100,12 @@ void host_feq(const HostLattice& lat, double density, const double* v, double* f
     }
 }
 #pragma clang fp contract(on)
+
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                int64_t n) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
 
 template <class T>
 __global__ void k_fill_planes(T* buf, int64_t plane, int q, const double* values) {
@@ -308,7 +315,14 @@ int lbm_domain_create(const lbm_desc* desc, lbm_domain** out) {
     L.off = A - 1;
     L.px = roundup(d->nx, A);
     L.pxy = L.px * d->ny;
-    L.plane = roundup(L.off + L.pxy * d->nz + 16, A);
+    // Planes are A-aligned; LBM_PLANE_ALIGN (elements) rounds them further
+    // (a tuning knob for HBM channel mapping of the Q concurrent streams).
+    int64_t palign = A;
+    if (const char* s = std::getenv("LBM_PLANE_ALIGN")) {
+        const long long v = std::atoll(s);
+        if (v > 0 && v % A == 0) palign = v;
+    }
+    L.plane = roundup(L.off + L.pxy * d->nz + 16, palign);
     L.xl = static_cast<int>(desc->xl);
     L.yl = static_cast<int>(desc->yl);
     L.zl = static_cast<int>(desc->zl);
@@ -702,6 +716,45 @@ int lbm_conservation(lbm_domain* d, double out[4]) {
                                 d->stream));
     HIPCHK(hipMemcpyAsync(out, d->d_out4, 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
+    return LBM_OK;
+}
+
+int lbm_copy_bandwidth(int device, uint64_t bytes, int reps, double out[2]) {
+    if (!out || bytes < 16 || reps < 1 || reps > 1000) return fail(LBM_ERR_ARG, "bad copy arguments");
+    HIPCHK(hipSetDevice(device));
+    const int64_t n = static_cast<int64_t>(bytes / 16);
+    void* a = nullptr;
+    void* b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return fail(LBM_ERR_ALLOC, "hipMalloc failed");
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(LBM_ERR_ALLOC, "hipMalloc failed");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemset(a, 0, n * 16);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const uint32_t grid = static_cast<uint32_t>((n + 255) / 256);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, nullptr, static_cast<const uint4*>(a),
+                           static_cast<uint4*>(b), n);
+        e = hipEventRecord(e0, nullptr);
+    }
+    for (int r = 0; e == hipSuccess && r < reps; ++r)
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, nullptr, static_cast<const uint4*>(a),
+                           static_cast<uint4*>(b), n);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("copy: ") + hipGetErrorString(e));
+    out[0] = ms / reps;
+    out[1] = 2.0 * static_cast<double>(n) * 16.0 / (out[0] * 1e-3) / 1e9;
     return LBM_OK;
 }
 

@@ -36,6 +36,52 @@ struct alignas(sizeof(T) * V) VecT {
     T v[V];
 };
 
+// Streaming (non-temporal) stores for the bulk output: every value written
+// in a step is read once in the next step, long after it would have been
+// evicted, so keeping it in L2 only displaces source lines.
+#ifndef LBM_NT_STORE
+#define LBM_NT_STORE 1
+#endif
+
+template <int BYTES>
+struct RawOf;
+template <>
+struct RawOf<1> { using type = uint8_t; };
+template <>
+struct RawOf<2> { using type = uint16_t; };
+template <>
+struct RawOf<4> { using type = uint32_t; };
+template <>
+struct RawOf<8> { using type = uint32_t __attribute__((ext_vector_type(2))); };
+template <>
+struct RawOf<16> { using type = uint32_t __attribute__((ext_vector_type(4))); };
+template <>
+struct RawOf<32> { using type = uint32_t __attribute__((ext_vector_type(8))); };
+
+template <class T, int V>
+__device__ __forceinline__ void store_stream(T* p, const VecT<T, V>& v) {
+#if LBM_NT_STORE
+    using R = typename RawOf<sizeof(T) * V>::type;
+    R r;
+    __builtin_memcpy(&r, &v, sizeof(R));
+    __builtin_nontemporal_store(r, reinterpret_cast<R*>(p));
+#else
+    *reinterpret_cast<VecT<T, V>*>(p) = v;
+#endif
+}
+
+template <class T>
+__device__ __forceinline__ void store_stream1(T* p, T v) {
+#if LBM_NT_STORE
+    using R = typename RawOf<sizeof(T)>::type;
+    R r;
+    __builtin_memcpy(&r, &v, sizeof(R));
+    __builtin_nontemporal_store(r, reinterpret_cast<R*>(p));
+#else
+    *p = v;
+#endif
+}
+
 __device__ __forceinline__ int wrap1(int v, int l) { return v == 0 ? l : (v == l + 1 ? 1 : v); }
 
 __device__ __forceinline__ int wrap_if(int v, int l, int per) {
@@ -161,11 +207,11 @@ __global__ void __launch_bounds__(256) k_bulk(StepArgs a, int nbx, int nby) {
             VecT<T, V> o;
 #pragma unroll
             for (int i = 0; i < V; ++i) o.v[i] = IO::template from_calc<q>(f[i][q]);
-            *reinterpret_cast<VecT<T, V>*>(out) = o;
+            store_stream<T, V>(out, o);
         } else {
 #pragma unroll
             for (int i = 0; i < V; ++i)
-                if (act[i]) out[i] = IO::template from_calc<q>(f[i][q]);
+                if (act[i]) store_stream1<T>(out + i, IO::template from_calc<q>(f[i][q]));
         }
     });
 }

@@ -6,9 +6,9 @@ namespace lbm_amd {
 
 namespace {
 using Lat = D3Q27;
-const ModeOps kF32 = OpsImpl<Lat, Mode::F32>::make<1, 2, 4>(4);
-const ModeOps kF16 = OpsImpl<Lat, Mode::F16>::make<2, 4, 8>(8);
-const ModeOps kBF16 = OpsImpl<Lat, Mode::BF16>::make<2, 4, 8>(8);
+const ModeOps kF32 = OpsImpl<Lat, Mode::F32>::make<1, 2, 4>(2);
+const ModeOps kF16 = OpsImpl<Lat, Mode::F16>::make<2, 4, 8>(4);
+const ModeOps kBF16 = OpsImpl<Lat, Mode::BF16>::make<2, 4, 8>(4);
 const ModeOps kF64S = OpsImpl<Lat, Mode::F64S>::make<1, 2>(1);
 }  // namespace
 

@@ -4,6 +4,7 @@
 // (the parity mode). The other modes keep the shifted value g_q = f_q - w_q,
 // multiplied by a per-direction power of two for the 16-bit formats, and do
 // their arithmetic in the Calc type (float, or double for F64S).
+// Conversions round to nearest even (v_cvt_f16_f32, v_cvt_pk_bf16_f32).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,23 +18,6 @@ namespace lbm_amd {
 
 enum class Mode : int { EXACT64 = 0, F32 = 1, F16 = 2, BF16 = 3, F64S = 4 };
 
-// 16-bit brain float kept as raw bits; conversions are written out so the
-// rounding is explicit (round to nearest, ties to even).
-struct bf16_t {
-    uint16_t bits;
-};
-
-__device__ __forceinline__ float bf16_to_float(uint16_t b) {
-    return __uint_as_float(static_cast<uint32_t>(b) << 16);
-}
-
-__device__ __forceinline__ uint16_t float_to_bf16(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);  // NaN
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return static_cast<uint16_t>(u >> 16);
-}
-
 template <Mode M>
 struct StorageOf;
 template <>
@@ -41,47 +25,42 @@ struct StorageOf<Mode::EXACT64> {
     using T = double;
     using Calc = double;
     static constexpr bool shifted = false;
-    static constexpr bool scaled = false;
 };
 template <>
 struct StorageOf<Mode::F64S> {
     using T = double;
     using Calc = double;
     static constexpr bool shifted = true;
-    static constexpr bool scaled = false;
 };
 template <>
 struct StorageOf<Mode::F32> {
     using T = float;
     using Calc = float;
     static constexpr bool shifted = true;
-    static constexpr bool scaled = false;
 };
 template <>
 struct StorageOf<Mode::F16> {
     using T = __half;
     using Calc = float;
     static constexpr bool shifted = true;
-    static constexpr bool scaled = true;
 };
 template <>
 struct StorageOf<Mode::BF16> {
-    using T = bf16_t;
+    using T = __bf16;
     using Calc = float;
     static constexpr bool shifted = true;
-    static constexpr bool scaled = true;
 };
 
 // Per-direction scale of the 16-bit formats: 2^8 times the power of two
 // nearest to 1/w_q. |g_q / w_q| stays below ~1 for stable flows, so stored
 // magnitudes sit near 2^8, far from the fp16 subnormal range (< 2^-14) and
-// with 2^8 of headroom below the fp16 maximum (65504).
+// with 2^8 of headroom below the fp16 maximum (65504). Powers of two keep the
+// scaling itself exact.
 template <class L>
 LBM_HD constexpr int scale_exp(int q) {
-    // nearest power of two to wden/wnum (ratio in [3.375, 216])
+    // smallest e with 2^e >= (wden / wnum) / sqrt(2), compared in integers
     int e = 0;
     const int num = L::wnum(q), den = L::wden;
-    // smallest e with 2^e * num >= den / sqrt(2): compare squares to stay in ints
     while (2 * (num << e) * (num << e) < den * den) ++e;
     return e + 8;
 }
@@ -96,19 +75,20 @@ LBM_HD constexpr float inv_scale_of(int q) {
     return 1.0f / static_cast<float>(1u << scale_exp<L>(q));
 }
 
-// Scalar load / store of one stored value as Calc, with the scale applied.
+// Load / store of one stored value as Calc, with the scale applied.
 template <Mode M, class L>
 struct Io {
     using S = StorageOf<M>;
     using T = typename S::T;
     using Calc = typename S::Calc;
+    static constexpr bool scaled = (M == Mode::F16 || M == Mode::BF16);
 
     template <int q>
     __device__ __forceinline__ static Calc to_calc(T v) {
         if constexpr (M == Mode::F16) {
             return __half2float(v) * inv_scale_of<L>(q);
         } else if constexpr (M == Mode::BF16) {
-            return bf16_to_float(v.bits) * inv_scale_of<L>(q);
+            return static_cast<float>(v) * inv_scale_of<L>(q);
         } else {
             return static_cast<Calc>(v);
         }
@@ -118,7 +98,7 @@ struct Io {
         if constexpr (M == Mode::F16) {
             return __float2half_rn(c * scale_of<L>(q));
         } else if constexpr (M == Mode::BF16) {
-            return bf16_t{float_to_bf16(c * scale_of<L>(q))};
+            return static_cast<__bf16>(c * scale_of<L>(q));
         } else {
             return static_cast<T>(c);
         }
@@ -128,18 +108,9 @@ struct Io {
         if constexpr (M == Mode::F16) {
             return __half2float(v) / static_cast<float>(1u << scale_exp<L>(q));
         } else if constexpr (M == Mode::BF16) {
-            return bf16_to_float(v.bits) / static_cast<float>(1u << scale_exp<L>(q));
+            return static_cast<float>(v) / static_cast<float>(1u << scale_exp<L>(q));
         } else {
             return static_cast<Calc>(v);
-        }
-    }
-    __device__ __forceinline__ static T from_calc_rt(Calc c, int q) {
-        if constexpr (M == Mode::F16) {
-            return __float2half_rn(c * static_cast<float>(1u << scale_exp<L>(q)));
-        } else if constexpr (M == Mode::BF16) {
-            return bf16_t{float_to_bf16(c * static_cast<float>(1u << scale_exp<L>(q)))};
-        } else {
-            return static_cast<T>(c);
         }
     }
 };

@@ -513,6 +513,19 @@ void ora_moments(const ora_domain* d, double* rho, double* u3) {
             }
 }
 
+void ora_init_equilibrium(ora_domain* d, const double* rho, const double* u3) {
+    double* field = d->field[d->cur];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t z = 1; z <= d->zl; ++z)
+        for (int64_t y = 1; y <= d->yl; ++y)
+            for (int64_t x = 1; x <= d->xl; ++x) {
+                const int64_t i = IDX(d, x, y, z);
+                if (!is_fluid_h(d->handler[i])) continue;
+                const int64_t o = (x - 1) + d->xl * ((y - 1) + d->yl * (z - 1));
+                compute_feq(d, rho[o], u3 + 3 * o, field + i * d->Q);
+            }
+}
+
 /* Mass and momentum summed over interior fluid cells in x-fastest order. */
 void ora_conservation(const ora_domain* d, double* out4) {
     const double* field = d->field[d->cur];

@@ -50,6 +50,7 @@ def load() -> ctypes.CDLL:
         lib.ora_moments.argtypes = [c_void_p, dp, dp]
         lib.ora_conservation.argtypes = [c_void_p, dp]
         lib.ora_get_kinds.argtypes = [c_void_p, POINTER(c_uint8)]
+        lib.ora_init_equilibrium.argtypes = [c_void_p, dp, dp]
         lib.ora_num_threads.restype = c_int
         _lib = lib
     return _lib
@@ -148,6 +149,12 @@ class OracleDomain:
         a = np.ascontiguousarray(f, dtype=np.float64)
         assert a.size == np.prod(self.ghost_shape) * self.q
         load().ora_set_pdf(self._h, _dp(a))
+
+    def init_equilibrium(self, rho, u):
+        r = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        assert r.size == self.xl * self.yl * self.zl and v.size == 3 * r.size
+        load().ora_init_equilibrium(self._h, _dp(r), _dp(v))
 
     def moments(self):
         rho = np.empty((self.zl, self.yl, self.xl))
