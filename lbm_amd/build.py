@@ -42,7 +42,7 @@ UNITS = {
     "mode_fast_q27.hip": [],
 }
 
-HEADERS = sorted(CSRC.glob("*.hpp")) + sorted(INCLUDE.glob("*.h"))
+HEADERS = sorted(CSRC.glob("*.hpp")) + sorted(INCLUDE.rglob("*.h*"))
 
 
 def _stale(obj: Path, src: Path) -> bool:
@@ -85,7 +85,24 @@ def build(force: bool = True, jobs: int | None = None, verbose: bool = False) ->
             print(out, file=sys.stderr)
             raise RuntimeError("link of liblbm_hip.so failed")
         os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+DRIVER = PKG / "lbm"
+
+
+def build_driver() -> Path:
+    """The drop-in `lbm` executable (host C++ over the C ABI, built with g++)."""
+    src = PKG / "apps" / "lbm_main.cpp"
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", str(src),
+           f"-L{PKG}", "-llbm_hip", "-Wl,-rpath,$ORIGIN", "-o", str(DRIVER)]
+    rc, out = _run(cmd)
+    if rc:
+        print(" ".join(cmd), file=sys.stderr)
+        print(out, file=sys.stderr)
+        raise RuntimeError("build of the lbm driver failed")
+    return DRIVER
 
 
 def build_oracle() -> Path:

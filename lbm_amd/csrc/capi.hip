@@ -17,7 +17,7 @@
 
 #include "lbm_hip.h"
 #include "domain_view.hpp"
-#include "lattice.hpp"
+#include "lbm_hip/lattice.hpp"
 #include "ops.hpp"
 #include "storage.hpp"
 

@@ -2,7 +2,7 @@
 #pragma once
 
 #include "domain_view.hpp"
-#include "lattice.hpp"
+#include "lbm_hip/lattice.hpp"
 #include "storage.hpp"
 
 namespace lbm_amd {

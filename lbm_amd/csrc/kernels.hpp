@@ -26,7 +26,7 @@
 
 #include "collide.hpp"
 #include "domain_view.hpp"
-#include "lattice.hpp"
+#include "lbm_hip/lattice.hpp"
 #include "storage.hpp"
 
 namespace lbm_amd {

@@ -6,7 +6,7 @@
 #include "collide.hpp"
 #include "domain_view.hpp"
 #include "kernels.hpp"
-#include "lattice.hpp"
+#include "lbm_hip/lattice.hpp"
 #include "storage.hpp"
 
 namespace lbm_amd {

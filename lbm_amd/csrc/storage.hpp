@@ -12,7 +12,7 @@
 
 #include <cstdint>
 
-#include "lattice.hpp"
+#include "lbm_hip/lattice.hpp"
 
 namespace lbm_amd {
 

@@ -1,0 +1,126 @@
+"""The drop-in `lbm` executable (lbm_amd/apps/lbm_main.cpp over include/lbm_hip/*.hpp):
+config + scenario parsing on the CPU, and on the GPU full runs whose .vts
+output is compared with the CPU oracle on the same scenario."""
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from lbm_amd import scenario
+
+ROOT = Path(__file__).resolve().parents[1]
+LBM = ROOT / "lbm_amd" / "lbm"
+GOLDEN = ROOT / "tests" / "golden" / "scenarios"
+
+
+def write_cfg(tmp: Path, scen: Path, steps: int, plot: int, extra: str = "") -> Path:
+    p = tmp / "config.cfg"
+    p.write_text(f"# test run\ncollision-model = bgk\ntau = 0.6\ntimesteps = {steps}\n"
+                 f"timesteps-per-plot = {plot}\noutput-dir = {tmp / 'vtk'}\nscenario-file = {scen}\n{extra}")
+    return p
+
+
+def run(args, cwd):
+    return subprocess.run([str(LBM), *map(str, args)], cwd=cwd, capture_output=True, text=True, timeout=600)
+
+
+@pytest.mark.parametrize("name", ["cavity.xml", "step.xml", "shearflow.xml"])
+def test_dry_run_lists_boundaries_in_file_order(tmp_path, name):
+    cfg = write_cfg(tmp_path, GOLDEN / name, 5, 0)
+    r = run(["--dry-run", cfg], tmp_path)
+    assert r.returncode == 0, r.stderr
+    got = re.findall(r"^boundary (.+) (\w+)$", r.stdout, flags=re.M)
+    sc = scenario.parse_scenario_file(GOLDEN / name)
+    assert got == [(b.extent, b.condition) for b in sc.boundaries]
+
+
+def test_config_errors(tmp_path):
+    cfg = write_cfg(tmp_path, GOLDEN / "cavity.xml", 5, 0)
+    r = run(["--dry-run", cfg, "--tau", "0.4"], tmp_path)
+    assert r.returncode == 1 and "An error occured: tau" in r.stderr
+    (tmp_path / "bad.cfg").write_text("tau = 0.6\ntimesteps = 3\n")
+    r = run(["--dry-run", tmp_path / "bad.cfg"], tmp_path)
+    assert r.returncode == 1 and "required but missing" in r.stderr
+    r = run([], tmp_path)
+    assert r.returncode == 1 and "Usage" in r.stdout
+
+
+def read_vts(path: Path):
+    text = path.read_text()
+    arrays = re.findall(r'<DataArray[^>]*Name="(\w+)"[^>]*>(.*?)</DataArray>', text, flags=re.S)
+    out = {name: np.array(body.split(), dtype=np.float64) for name, body in arrays}
+    return out["Density"], out["Velocity"].reshape(-1, 3)
+
+
+def oracle_moments(sc, steps, q=19, tau=0.6):
+    from oracle import ora
+    o = ora.OracleDomain(sc.xl, sc.yl, sc.zl, tau, q=q)
+    sc.apply(o)
+    o.finalize()
+    o.step(steps)
+    rho, u = o.moments()
+    fluid = o.kinds()[1:-1, 1:-1, 1:-1] == 0
+    return rho.reshape(-1), u.reshape(-1, 3), fluid.reshape(-1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,lattice,q", [("cavity16", "D3Q19", 19), ("step.xml", "D3Q19", 19),
+                                            ("shearflow.xml", "D3Q27", 27), ("step.xml", "D3Q15", 15)])
+def test_driver_matches_oracle(tmp_path, name, lattice, q):
+    print(f"driver {name} {lattice}", flush=True)
+    if name == "cavity16":
+        scen = tmp_path / "cavity16.xml"
+        scen.write_text((GOLDEN / "cavity.xml").read_text().replace('xl="40" yl="40" zl="40"',
+                                                                     'xl="16" yl="12" zl="14"'))
+    else:
+        scen = GOLDEN / name
+    steps = 30
+    cfg = write_cfg(tmp_path, scen, steps, 15, f"lattice = {lattice}\n")
+    r = run([cfg], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "MLUPS:" in r.stdout and "Finished!" in r.stdout
+    sc = scenario.parse_scenario_file(scen)
+    rho, u = read_vts(tmp_path / "vtk" / f"{sc.name}.{steps}.vts")
+    ro, uo, fluid = oracle_moments(sc, steps, q=q)
+    assert np.array_equal(rho[fluid], ro[fluid])
+    assert np.array_equal(u[fluid], uo[fluid])
+    assert (tmp_path / "vtk" / f"{sc.name}.15.vts").exists()
+
+
+@pytest.mark.gpu
+def test_driver_vtk_mask_scenario(tmp_path):
+    """A pipe through a fluid mask (legacy STRUCTURED_POINTS file) with inflow
+    and outflow, as build/scenarios/pipe.xml sets up."""
+    print("driver mask pipe", flush=True)
+    xl, yl, zl = 20, 9, 9
+    z, y, x = np.meshgrid(np.arange(zl), np.arange(yl), np.arange(xl), indexing="ij")
+    mask = (((y - 4) ** 2 + (z - 4) ** 2) <= 12).astype(np.uint8)
+    lines = ["# vtk DataFile Version 2.0", "pipe mask", "ASCII", "DATASET STRUCTURED_POINTS",
+             f"DIMENSIONS {xl} {yl} {zl}", "ORIGIN 0 0 0", "SPACING 1 1 1", f"POINT_DATA {mask.size}",
+             "SCALARS inputfluidMask unsigned_char", "LOOKUP_TABLE default"]
+    lines += [" ".join(map(str, mask.reshape(-1)[i:i + 20])) for i in range(0, mask.size, 20)]
+    (tmp_path / "pipe.vtk").write_text("\n".join(lines) + "\n")
+    scen = tmp_path / "pipe.xml"
+    scen.write_text(f"""<?xml version="1.0" ?>
+<scenario name="Pipe flow">
+   <domain vtk-file="{tmp_path / 'pipe.vtk'}">
+        <boundary extent="z0" condition="noslip" />
+        <boundary extent="zmax" condition="noslip" />
+        <boundary extent="x0" condition="inflow" vx="0.03" vy="0" vz="0" />
+        <boundary extent="xmax" condition="outflow" />
+        <boundary extent="y0" condition="noslip" />
+        <boundary extent="ymax" condition="noslip" />
+    </domain>
+</scenario>
+""")
+    steps = 40
+    cfg = write_cfg(tmp_path, scen, steps, steps)
+    r = run([cfg], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    sc = scenario.parse_scenario_file(scen)
+    rho, u = read_vts(tmp_path / "vtk" / f"Pipe flow.{steps}.vts")
+    ro, uo, fluid = oracle_moments(sc, steps)
+    assert fluid.sum() < fluid.size  # the mask made solids
+    assert np.array_equal(rho[fluid], ro[fluid]) and np.array_equal(u[fluid], uo[fluid])
