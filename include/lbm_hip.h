@@ -160,6 +160,49 @@ int lbm_time_steps(lbm_domain* d, uint64_t nsteps, double out[4]);
  * reference next to the roofline. */
 int lbm_copy_bandwidth(int device, uint64_t bytes, int reps, double out[2]);
 
+/* ---------------------------------------------------------------------------
+ * Differentiation (extension; the reference has none). A tape records one
+ * forward run of `nsteps` steps from the domain's state at lbm_adj_create
+ * and returns the objective; lbm_adj_backward runs the hand-written adjoint
+ * step backwards through it (sqrt(N) checkpoints of the distribution field,
+ * segments recomputed on the device) and returns fp64 gradients with respect
+ * to the Brinkman porosity of every interior cell and to the velocity of the
+ * moving walls (summed over all MovingWall conditions). The pair has the
+ * shape of a torch.autograd.Function: forward(params) -> J,
+ * backward(dJ) -> dJ * grad. Shifted storage modes (F32, F64S), D3Q19, with
+ * NoSlip, MovingWall and Inflow boundaries. The domain must be finalized.
+ */
+enum lbm_objective {
+    LBM_OBJ_PLANE_VELOCITY_MISFIT = 1, /* sum over cells with x == plane_x of |u - u_target|^2 */
+    LBM_OBJ_KINETIC_ENERGY = 2         /* 1/2 sum over fluid cells of |u|^2 */
+};
+
+typedef struct lbm_adj_desc {
+    int objective;          /* enum lbm_objective */
+    int plane_x;            /* interior x of the misfit plane (1..xl) */
+    const double* target_u; /* yl*zl*3 target velocities of the plane, y fastest (copied) */
+    uint64_t nsteps;        /* steps of the forward run */
+    uint64_t checkpoint_every; /* 0: ceil(sqrt(nsteps)) */
+} lbm_adj_desc;
+
+typedef struct lbm_tape lbm_tape;
+
+int lbm_adj_create(lbm_domain* d, const lbm_adj_desc* desc, lbm_tape** out);
+int lbm_adj_destroy(lbm_tape* t);
+/* Forward: alpha (xl*yl*zl, x fastest) replaces the domain's porosity when
+ * not NULL (the domain needs a Brinkman field then); J receives the objective.
+ * Leaves the final state in the domain. */
+int lbm_adj_forward(lbm_tape* t, const double* alpha, double* J);
+/* Backward after a forward: grad_alpha (xl*yl*zl, or NULL) and grad_wall[3]
+ * (or NULL) receive dJ times the gradients. */
+int lbm_adj_backward(lbm_tape* t, double dJ, double* grad_alpha, double grad_wall[3]);
+/* fp64 forward-mode directional derivative along dalpha (dual numbers through
+ * the same step kernels): J and dJ/ds of J(alpha + s dalpha) at s = 0. */
+int lbm_jvp(lbm_tape* t, const double* alpha, const double* dalpha, double* J, double* dJ);
+/* Timing of the last backward: out[0] ms in recompute steps, out[1] ms in
+ * adjoint steps, out[2] forward ms of the last lbm_adj_forward, out[3] steps. */
+int lbm_adj_timing(lbm_tape* t, double out[4]);
+
 /* Layout and size information. */
 typedef struct lbm_info {
     uint64_t ncells_interior;  /* xl*yl*zl */

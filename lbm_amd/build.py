@@ -36,6 +36,8 @@ COMMON = [
 # translation unit -> extra flags
 UNITS = {
     "capi.hip": [],
+    "capi_adjoint.hip": [],
+    "mode_adjoint.hip": [],
     "mode_exact.hip": ["-ffp-contract=off"],
     "mode_fast_q15.hip": [],
     "mode_fast_q19.hip": [],

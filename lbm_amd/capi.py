@@ -69,6 +69,13 @@ SIGNATURES = [
     ("lbm_conservation", c_int, [c_void_p, POINTER(c_double)]),
     ("lbm_time_steps", c_int, [c_void_p, c_uint64, POINTER(c_double)]),
     ("lbm_copy_bandwidth", c_int, [c_int, c_uint64, c_int, POINTER(c_double)]),
+    ("lbm_adj_create", c_int, [c_void_p, c_void_p, POINTER(c_void_p)]),
+    ("lbm_adj_destroy", c_int, [c_void_p]),
+    ("lbm_adj_forward", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+    ("lbm_adj_backward", c_int, [c_void_p, c_double, POINTER(c_double), POINTER(c_double)]),
+    ("lbm_jvp", c_int, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                        POINTER(c_double)]),
+    ("lbm_adj_timing", c_int, [c_void_p, POINTER(c_double)]),
     ("lbm_get_info", c_int, [c_void_p, POINTER(Info)]),
     ("lbm_set_tuning", c_int, [c_void_p, c_int, c_int]),
 ]
@@ -235,3 +242,76 @@ class Domain:
 
     def set_tuning(self, vec: int = 0, block_x: int = 0) -> None:
         _check(load().lbm_set_tuning(self._h, vec, block_x))
+
+
+class AdjDesc(ctypes.Structure):
+    _fields_ = [("objective", c_int), ("plane_x", c_int), ("target_u", POINTER(c_double)),
+                ("nsteps", c_uint64), ("checkpoint_every", c_uint64)]
+
+
+OBJ_PLANE_VELOCITY_MISFIT, OBJ_KINETIC_ENERGY = 1, 2
+
+
+class Tape:
+    """Differentiation tape over a finalized Domain (lbm_adj_* / lbm_jvp).
+
+    forward(alpha) -> J, backward(dJ) -> (grad_alpha, grad_wall), jvp(alpha,
+    dalpha) -> (J, dJ). alpha / grads are interior arrays (zl, yl, xl)."""
+
+    def __init__(self, dom: Domain, nsteps: int, objective: int = OBJ_KINETIC_ENERGY,
+                 plane_x: int = 0, target_u=None, checkpoint_every: int = 0):
+        self.dom = dom
+        self._target = None
+        d = AdjDesc()
+        d.objective, d.plane_x, d.nsteps, d.checkpoint_every = objective, plane_x, nsteps, checkpoint_every
+        if target_u is not None:
+            self._target = np.ascontiguousarray(target_u, dtype=np.float64).reshape(-1)
+            assert self._target.size == dom.yl * dom.zl * 3
+            d.target_u = _dptr(self._target)
+        h = c_void_p()
+        _check(load().lbm_adj_create(dom._h, byref(d), byref(h)))
+        self._h = h
+        self.nsteps = nsteps
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().lbm_adj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _alpha(self, alpha):
+        if alpha is None:
+            return None, None
+        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1)
+        assert a.size == self.dom.xl * self.dom.yl * self.dom.zl
+        return a, _dptr(a)
+
+    def forward(self, alpha=None) -> float:
+        a, p = self._alpha(alpha)
+        j = c_double()
+        _check(load().lbm_adj_forward(self._h, p, byref(j)))
+        return j.value
+
+    def backward(self, dJ: float = 1.0, want_alpha: bool = True):
+        shape = (self.dom.zl, self.dom.yl, self.dom.xl)
+        g = np.zeros(shape) if want_alpha else None
+        gw = np.zeros(3)
+        _check(load().lbm_adj_backward(self._h, dJ, _dptr(g) if want_alpha else None, _dptr(gw)))
+        return g, gw
+
+    def jvp(self, alpha, dalpha):
+        a, p = self._alpha(alpha)
+        da = np.ascontiguousarray(dalpha, dtype=np.float64).reshape(-1)
+        j, dj = c_double(), c_double()
+        _check(load().lbm_jvp(self._h, p, _dptr(da), byref(j), byref(dj)))
+        return j.value, dj.value
+
+    def timing(self) -> dict:
+        out = np.zeros(4)
+        _check(load().lbm_adj_timing(self._h, _dptr(out)))
+        return {"recompute_ms": out[0], "adjoint_ms": out[1], "forward_ms": out[2], "steps": int(out[3])}

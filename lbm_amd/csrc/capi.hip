@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "domain_impl.hpp"
 #include "lbm_hip.h"
 #include "domain_view.hpp"
 #include "lbm_hip/lattice.hpp"
@@ -23,64 +24,15 @@
 
 using namespace lbm_amd;
 
+
+namespace lbm_amd {
+std::string& last_error() {
+    thread_local std::string e;
+    return e;
+}
+}  // namespace lbm_amd
+
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-#define HIPCHK(expr)                                                                  \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
-        if (e_ != hipSuccess)                                                         \
-            return fail(LBM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-struct HostCond {
-    int kind;
-    double vel[3];
-    double rho;
-};
-
-// Lattice tables as plain arrays for host code.
-struct HostLattice {
-    int Q = 0;
-    int c[27][3];
-    double w[27];
-};
-
-template <class L>
-HostLattice make_lattice() {
-    HostLattice h;
-    h.Q = L::Q;
-    for (int q = 0; q < L::Q; ++q) {
-        h.c[q][0] = L::cx(q);
-        h.c[q][1] = L::cy(q);
-        h.c[q][2] = L::cz(q);
-        h.w[q] = weight<L>(q);
-    }
-    return h;
-}
-
-HostLattice lattice_for(int q) {
-    if (q == 15) return make_lattice<D3Q15>();
-    if (q == 19) return make_lattice<D3Q19>();
-    return make_lattice<D3Q27>();
-}
-
-size_t elem_size(Mode m) {
-    switch (m) {
-        case Mode::EXACT64:
-        case Mode::F64S: return 8;
-        case Mode::F32: return 4;
-        default: return 2;
-    }
-}
-
-int64_t roundup(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
 // Reference equilibrium (include/collision.hpp:34-51) on the host, for the
 // constant Inflow table.
@@ -117,80 +69,7 @@ __global__ void k_fill_planes(T* buf, int64_t plane, int q, const double* values
 
 }  // namespace
 
-struct lbm_domain {
-    lbm_desc desc{};
-    Mode mode = Mode::EXACT64;
-    const ModeOps* ops = nullptr;
-    HostLattice lat;
-    int Q = 0;
-    int nx = 0, ny = 0, nz = 0;
-    int64_t ncell = 0;
-    FieldLayout L{};
-    size_t esize = 0;
-    size_t field_bytes = 0;
-    void* buf[2] = {nullptr, nullptr};
-    int cur = 0;
-    uint8_t* d_kind = nullptr;
-    CellCoord* d_blist = nullptr;
-    int64_t nb = 0;
-    Cond* d_conds = nullptr;
-    int nconds_dev = 0;
-    void* d_alpha = nullptr;
-    double g[3] = {0, 0, 0};
-    std::vector<uint16_t> handler;  // per ghosted cell: 0 fluid, else 1 + index in conds
-    std::vector<HostCond> conds;
-    int mask_cond = -1;
-    std::vector<double> h_alpha;  // interior, x fastest (empty: none)
-    bool finalized = false;
-    bool materialised = true;
-    hipStream_t stream = nullptr;
-    StepPlan plan{};
-    int tune_vec = 0, tune_bx = 0;
-    double* d_partial = nullptr;
-    double* d_out4 = nullptr;
-    int nred = 0;
-    uint64_t n_fluid = 0;
-    size_t device_bytes = 0;
-
-    int64_t hidx(int64_t x, int64_t y, int64_t z) const { return x + nx * (y + ny * z); }
-    bool interior(int64_t x, int64_t y, int64_t z) const {
-        return x >= 1 && x <= L.xl && y >= 1 && y <= L.yl && z >= 1 && z <= L.zl;
-    }
-    int64_t wrapc(int64_t v, int axis) const {
-        const int64_t l = axis == 0 ? L.xl : (axis == 1 ? L.yl : L.zl);
-        if (!L.per[axis]) return v;
-        if (v < 1) return v + l;
-        if (v > l) return v - l;
-        return v;
-    }
-    StepArgs args(const void* src, void* dst, int mat) const {
-        StepArgs a{};
-        a.L = L;
-        a.src = src;
-        a.dst = dst;
-        a.kind = d_kind;
-        a.blist = d_blist;
-        a.nb = nb;
-        a.conds = d_conds;
-        a.materialised = mat;
-        a.tau = desc.tau;
-        a.omega = 1.0 / desc.tau;
-        a.alpha = d_alpha;
-        a.g[0] = g[0];
-        a.g[1] = g[1];
-        a.g[2] = g[2];
-        return a;
-    }
-};
-
 namespace {
-
-int check_handle(lbm_domain* d) {
-    if (!d) return fail(LBM_ERR_ARG, "null domain handle");
-    hipError_t e = hipSetDevice(d->desc.device);
-    if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
-    return LBM_OK;
-}
 
 int make_plan(lbm_domain* d) {
     const ModeOps* o = d->ops;
@@ -255,7 +134,7 @@ int fill_initial(lbm_domain* d, void* buf) {
 
 extern "C" {
 
-const char* lbm_last_error(void) { return g_err.c_str(); }
+const char* lbm_last_error(void) { return lbm_amd::last_error().c_str(); }
 int lbm_abi_version(void) { return LBM_HIP_ABI_VERSION; }
 
 int lbm_lattice(int q, int* c, double* w, double* cs) {
@@ -346,7 +225,7 @@ int lbm_domain_create(const lbm_desc* desc, lbm_domain** out) {
     d->device_bytes = 2 * d->field_bytes + static_cast<size_t>(L.plane);
     for (int k = 0; k < 2; ++k) {
         int rc = fill_initial(d, d->buf[k]);
-        if (rc) return bail(rc, g_err);
+        if (rc) return bail(rc, last_error());
     }
     if (hipStreamSynchronize(d->stream) != hipSuccess) return bail(LBM_ERR_HIP, "initial fill failed");
     *out = d;
@@ -547,18 +426,7 @@ int lbm_finalize(lbm_domain* d) {
     }
     d->nconds_dev = static_cast<int>(table.size());
     if (!d->h_alpha.empty()) {
-        const size_t cs = (d->mode == Mode::F64S) ? 8 : 4;
-        std::vector<char> a(static_cast<size_t>(L.plane) * cs, 0);
-        int64_t i = 0;
-        for (int z = 1; z <= L.zl; ++z)
-            for (int y = 1; y <= L.yl; ++y)
-                for (int x = 1; x <= L.xl; ++x, ++i) {
-                    const int64_t e = cell_elem(L, x, y, z);
-                    if (cs == 8)
-                        reinterpret_cast<double*>(a.data())[e] = d->h_alpha[i];
-                    else
-                        reinterpret_cast<float*>(a.data())[e] = static_cast<float>(d->h_alpha[i]);
-                }
+        const std::vector<char> a = alpha_in_layout(d, d->h_alpha.data());
         HIPCHK(hipMalloc(&d->d_alpha, a.size()));
         HIPCHK(hipMemcpyAsync(d->d_alpha, a.data(), a.size(), hipMemcpyHostToDevice, d->stream));
         d->device_bytes += a.size();

@@ -12,11 +12,14 @@
 
 #include <cstdint>
 
+#include "dual.hpp"
 #include "lbm_hip/lattice.hpp"
 
 namespace lbm_amd {
 
-enum class Mode : int { EXACT64 = 0, F32 = 1, F16 = 2, BF16 = 3, F64S = 4 };
+// DUAL64 is internal: shifted values as (value, tangent) fp64 pairs for the
+// forward-mode derivative run.
+enum class Mode : int { EXACT64 = 0, F32 = 1, F16 = 2, BF16 = 3, F64S = 4, DUAL64 = 5 };
 
 template <Mode M>
 struct StorageOf;
@@ -30,6 +33,12 @@ template <>
 struct StorageOf<Mode::F64S> {
     using T = double;
     using Calc = double;
+    static constexpr bool shifted = true;
+};
+template <>
+struct StorageOf<Mode::DUAL64> {
+    using T = Dual<double>;
+    using Calc = Dual<double>;
     static constexpr bool shifted = true;
 };
 template <>
