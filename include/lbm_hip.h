@@ -203,6 +203,43 @@ int lbm_jvp(lbm_tape* t, const double* alpha, const double* dalpha, double* J, d
  * adjoint steps, out[2] forward ms of the last lbm_adj_forward, out[3] steps. */
 int lbm_adj_timing(lbm_tape* t, double out[4]);
 
+/* ---------------------------------------------------------------------------
+ * Batching (extension; the reference runs one domain per process). A batch
+ * steps many finalized domains of any sizes and relaxation times together:
+ * one persistent bulk launch walks a tile work list over all domains and one
+ * launch covers all near-boundary cells; lbm_batch_step captures the whole
+ * run of nsteps as a hipGraph and launches it once. Results are bit-identical
+ * to stepping each domain alone. All domains share the lattice and storage
+ * (D3Q19; F32, F16, BF16 or F64S), the device, and their step parity; the
+ * domains stay owned by the caller and must outlive the batch. Fields and
+ * moments are read through the individual domain handles after lbm_batch_sync.
+ */
+typedef struct lbm_batch lbm_batch;
+
+int lbm_batch_create(lbm_domain* const* domains, int n, lbm_batch** out);
+int lbm_batch_destroy(lbm_batch* b);
+/* Autotuner: times `trial_steps` steps of every candidate launch shape
+ * (cells per thread, threads along x per tile, persistent blocks per CU) and
+ * keeps the fastest; best_ms receives its time per step. */
+int lbm_batch_tune(lbm_batch* b, int trial_steps, double* best_ms);
+int lbm_batch_step(lbm_batch* b, uint64_t nsteps);
+int lbm_batch_sync(lbm_batch* b);
+/* Times one graph launch of nsteps with HIP events: out[0] = ms,
+ * out[1] = interior cells of all domains (for MLUPS). */
+int lbm_batch_time_steps(lbm_batch* b, uint64_t nsteps, double out[2]);
+
+typedef struct lbm_batch_info {
+    uint64_t ndomains;
+    uint64_t ncells;   /* interior cells of all domains */
+    uint64_t ntiles;
+    uint64_t nnear;
+    int vec;           /* cells per thread */
+    int blocks_per_cu; /* persistent blocks per CU (0: one block per tile) */
+    int bx_policy;     /* 0: smallest power of two covering a row, 1: half of it */
+    int ncu;
+} lbm_batch_info;
+int lbm_batch_get_info(lbm_batch* b, lbm_batch_info* info);
+
 /* Layout and size information. */
 typedef struct lbm_info {
     uint64_t ncells_interior;  /* xl*yl*zl */

@@ -33,15 +33,21 @@ COMMON = [
     "-Wall", "-Wno-unused-function",
 ]
 
-# translation unit -> extra flags
+# translation unit -> extra flags. Kernels are built without implicit FMA
+# contraction: every product and sum is rounded as written, so a cell's result
+# does not depend on how many cells a thread handles or which kernel (single,
+# batched) handles it.
+NOFMA = ["-ffp-contract=off"]
 UNITS = {
     "capi.hip": [],
     "capi_adjoint.hip": [],
-    "mode_adjoint.hip": [],
-    "mode_exact.hip": ["-ffp-contract=off"],
-    "mode_fast_q15.hip": [],
-    "mode_fast_q19.hip": [],
-    "mode_fast_q27.hip": [],
+    "mode_adjoint.hip": NOFMA,
+    "capi_batch.hip": [],
+    "mode_batch.hip": NOFMA,
+    "mode_exact.hip": NOFMA,
+    "mode_fast_q15.hip": NOFMA,
+    "mode_fast_q19.hip": NOFMA,
+    "mode_fast_q27.hip": NOFMA,
 }
 
 HEADERS = sorted(CSRC.glob("*.hpp")) + sorted(INCLUDE.rglob("*.h*"))

@@ -76,6 +76,13 @@ SIGNATURES = [
     ("lbm_jvp", c_int, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double),
                         POINTER(c_double)]),
     ("lbm_adj_timing", c_int, [c_void_p, POINTER(c_double)]),
+    ("lbm_batch_create", c_int, [POINTER(c_void_p), c_int, POINTER(c_void_p)]),
+    ("lbm_batch_destroy", c_int, [c_void_p]),
+    ("lbm_batch_tune", c_int, [c_void_p, c_int, POINTER(c_double)]),
+    ("lbm_batch_step", c_int, [c_void_p, c_uint64]),
+    ("lbm_batch_sync", c_int, [c_void_p]),
+    ("lbm_batch_time_steps", c_int, [c_void_p, c_uint64, POINTER(c_double)]),
+    ("lbm_batch_get_info", c_int, [c_void_p, c_void_p]),
     ("lbm_get_info", c_int, [c_void_p, POINTER(Info)]),
     ("lbm_set_tuning", c_int, [c_void_p, c_int, c_int]),
 ]
@@ -315,3 +322,53 @@ class Tape:
         out = np.zeros(4)
         _check(load().lbm_adj_timing(self._h, _dptr(out)))
         return {"recompute_ms": out[0], "adjoint_ms": out[1], "forward_ms": out[2], "steps": int(out[3])}
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("ndomains", c_uint64), ("ncells", c_uint64), ("ntiles", c_uint64),
+                ("nnear", c_uint64), ("vec", c_int), ("blocks_per_cu", c_int),
+                ("bx_policy", c_int), ("ncu", c_int)]
+
+
+class Batch:
+    """Batched stepping of finalized domains (lbm_batch_*). The domains stay
+    owned by the caller (kept alive here)."""
+
+    def __init__(self, domains):
+        self.domains = list(domains)
+        arr = (c_void_p * len(self.domains))(*[d._h.value for d in self.domains])
+        h = c_void_p()
+        _check(load().lbm_batch_create(arr, len(self.domains), byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().lbm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tune(self, trial_steps: int = 5) -> float:
+        ms = c_double()
+        _check(load().lbm_batch_tune(self._h, trial_steps, byref(ms)))
+        return ms.value
+
+    def step(self, n: int) -> None:
+        _check(load().lbm_batch_step(self._h, n))
+
+    def sync(self) -> None:
+        _check(load().lbm_batch_sync(self._h))
+
+    def time_steps(self, n: int) -> dict:
+        out = np.zeros(2)
+        _check(load().lbm_batch_time_steps(self._h, n, _dptr(out)))
+        return {"ms": float(out[0]), "cells": int(out[1]), "mlups": out[1] * n / (out[0] * 1e3)}
+
+    def info(self) -> dict:
+        i = BatchInfo()
+        _check(load().lbm_batch_get_info(self._h, ctypes.cast(byref(i), c_void_p)))
+        return {k: getattr(i, k) for k, _ in BatchInfo._fields_}

@@ -95,22 +95,14 @@ __device__ __forceinline__ int wrap_if(int v, int l, int per) {
 // Source rows are wrapped unconditionally: along a periodic axis that is the
 // periodic image, and along a walled axis any cell next to the wall pulls
 // from a ghost cell, is NEAR and is masked out before its values are used.
+// V cells (x0 .. x0+V-1, y, z) of one thread.
 template <class L, Mode M, int V, bool FLAGS, bool BRINK>
-__global__ void __launch_bounds__(256) k_bulk(StepArgs a, int nbx, int nby) {
+__device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int z) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
     constexpr int Q = L::Q;
     const FieldLayout& Ly = a.L;
-
-    const int b = blockIdx.x;
-    const int bx = b % nbx;
-    const int t = b / nbx;
-    const int by = t % nby;
-    const int z = 1 + t / nby;
-    const int xv = bx * blockDim.x + threadIdx.x;
-    const int y = 1 + by * blockDim.y + threadIdx.y;
-    const int x0 = 1 + xv * V;
     if (x0 > Ly.xl || y > Ly.yl) return;
 
     const int64_t e0 = cell_elem(Ly, x0, y, z);
@@ -214,6 +206,18 @@ __global__ void __launch_bounds__(256) k_bulk(StepArgs a, int nbx, int nby) {
                 if (act[i]) store_stream1<T>(out + i, IO::template from_calc<q>(f[i][q]));
         }
     });
+}
+
+template <class L, Mode M, int V, bool FLAGS, bool BRINK>
+__global__ void __launch_bounds__(256) k_bulk(StepArgs a, int nbx, int nby) {
+    const int b = blockIdx.x;
+    const int bx = b % nbx;
+    const int t = b / nbx;
+    const int by = t % nby;
+    const int z = 1 + t / nby;
+    const int xv = bx * blockDim.x + threadIdx.x;
+    const int y = 1 + by * blockDim.y + threadIdx.y;
+    bulk_cells<L, M, V, FLAGS, BRINK>(a, 1 + xv * V, y, z);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,18 +355,14 @@ __device__ __forceinline__ typename Io<M, L>::Calc pull_nonfluid(
 }
 
 // ---------------------------------------------------------------------------
-// Near-boundary kernel: one thread per listed NEAR cell.
+// One NEAR cell (fluid with a non-fluid source).
 template <class L, Mode M>
-__global__ void __launch_bounds__(128) k_near(StepArgs a) {
+__device__ __forceinline__ void near_cell(const StepArgs& a, int x, int y, int z) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
     constexpr int Q = L::Q;
     const FieldLayout& Ly = a.L;
-    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (i >= a.nb) return;
-    const CellCoord c = a.blist[i];
-    const int x = c.x, y = c.y, z = c.z;
     const int64_t e = cell_elem(Ly, x, y, z);
     const T* __restrict__ src = static_cast<const T*>(a.src);
     T* __restrict__ dst = static_cast<T*>(a.dst);
@@ -410,6 +410,15 @@ __global__ void __launch_bounds__(128) k_near(StepArgs a) {
         constexpr int q = decltype(qc)::value;
         dst[q * Ly.plane + e] = IO::template from_calc<q>(f[q]);
     });
+}
+
+// Near-boundary kernel: one thread per listed NEAR cell.
+template <class L, Mode M>
+__global__ void __launch_bounds__(128) k_near(StepArgs a) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= a.nb) return;
+    const CellCoord c = a.blist[i];
+    near_cell<L, M>(a, c.x, c.y, c.z);
 }
 
 }  // namespace lbm_amd

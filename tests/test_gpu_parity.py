@@ -277,6 +277,37 @@ def test_shifted_periodic_tgv_error(storage):
     assert eu[0] < SHIFTED_TOL[storage]
 
 
+VECS = {"f64": (1, 2), "f64s": (1, 2), "f32": (1, 2, 4), "f16": (2, 4, 8), "bf16": (2, 4, 8)}
+
+
+@pytest.mark.parametrize("storage", ["f64", "f64s", "f32", "f16", "bf16"])
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_results_independent_of_cells_per_thread(storage, q):
+    """Kernels round every product and sum as written (no FMA contraction), so
+    the number of cells a thread handles and the block width do not change a
+    bit of the result: a walled cavity (bulk and boundary kernels) and a
+    periodic Taylor-Green box with an odd x length (partial vectors)."""
+    say(f"V independence {storage} q={q}")
+    runs = {}
+    for v in VECS[storage]:
+        for bx in (0, 64):
+            cav = capi.Domain(13, 11, 9, 0.6, q=q, storage=storage)
+            scenario.cavity(13, 11, 9).apply(cav)
+            cav.finalize()
+            per = capi.Domain(21, 8, 6, 0.8, q=q, storage=storage, periodic=(1, 1, 1))
+            per.finalize()
+            rho, u = tgv_fields(21, 8, 6, u0=0.03)
+            per.init_equilibrium(rho, u)
+            for d in (cav, per):
+                d.set_tuning(v, bx)
+                d.step(23)
+            runs[(v, bx)] = (cav.get_pdf(), per.get_pdf())
+    ref = runs[(VECS[storage][0], 0)]
+    for key, (fc, fp) in runs.items():
+        assert np.array_equal(fc.view(np.uint64), ref[0].view(np.uint64)), key
+        assert np.array_equal(fp.view(np.uint64), ref[1].view(np.uint64)), key
+
+
 # --------------------------------------------------- full-size properties --
 @pytest.mark.parametrize("q", [15, 19, 27])
 def test_tgv_256_decay_and_mass(q):
