@@ -10,8 +10,18 @@ Prints one JSON line: MLUPS of the fp32 mode (interior cells x steps / s),
 the roofline of the dominant kernel (algorithmic bytes 2*Q*sizeof(storage)
 per cell over its HIP-event-timed average launch), the fp16/bf16 modes and a
 CPU baseline (the fp64 CPU oracle, OpenMP on the host cores, bounded sample).
+Two further legs ride on the same line:
+- "adjoint": the inverse problem on a periodic 128^3 D3Q19 fp32 box with a
+  Brinkman porosity field (seeded uniform 32^3, trilinear), body force 1e-5 x,
+  2000 steps with sqrt(N) checkpoints, plane-velocity misfit objective: wall
+  time of the forward pass, of the checkpoint recompute and of the adjoint
+  sweep, and their ratio;
+- "batch": 512 lid-driven D3Q19 fp32 cavities (edges log-uniform in
+  [16, 96], Re uniform in [10, 1000] through per-domain tau) stepped 2000
+  times in one persistent, graph-captured launch per step after autotuning.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--modes f32,f16,bf16]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-adjoint] [--no-batch]
+                    [--modes f32,f16,bf16]
 """
 from __future__ import annotations
 
@@ -106,6 +116,62 @@ def cpu_baseline(rho, u, steps: int = 4):
                       f"{dt:.1f} s"}
 
 
+def adjoint_leg(capi, steps: int = 2000):
+    """Forward + recompute + adjoint of the 128^3 Brinkman inverse problem."""
+    from lbm_amd import workloads
+    n, tau, g = 128, 0.8, (1e-5, 0.0, 0.0)
+    _, alpha, alpha_t = workloads.porosity_pair(n)
+    px = n // 2
+    dt = capi.Domain(n, n, n, tau, q=Q, storage="f32", periodic=(True, True, True))
+    dt.set_brinkman(alpha_t, g)
+    dt.finalize()
+    dt.step(steps)
+    _, ut = dt.moments()
+    target = np.ascontiguousarray(ut[:, :, px - 1, :])
+    dt.close()
+    d = capi.Domain(n, n, n, tau, q=Q, storage="f32", periodic=(True, True, True))
+    d.set_brinkman(alpha, g)
+    d.finalize()
+    tape = capi.Tape(d, steps, capi.OBJ_PLANE_VELOCITY_MISFIT, plane_x=px, target_u=target)
+    tape.forward(alpha)
+    tape.backward(1.0)
+    tm = tape.timing()
+    tape.close()
+    d.close()
+    fwd = tm["forward_ms"]
+    return {"workload": "D3Q19 periodic 128^3 Brinkman, g=1e-5 x, tau=0.8, fp32, plane-misfit objective",
+            "steps": steps, "checkpoints": "sqrt(N)",
+            "forward_ms": round(fwd, 3), "recompute_ms": round(tm["recompute_ms"], 3),
+            "adjoint_ms": round(tm["adjoint_ms"], 3),
+            "adjoint_step_over_forward_step": round(tm["adjoint_ms"] / fwd, 3),
+            "backward_over_forward": round((tm["recompute_ms"] + tm["adjoint_ms"]) / fwd, 3)}
+
+
+def batch_leg(capi, steps: int = 2000, n: int = 512):
+    """512 uneven lid-driven cavities in one persistent graph-captured launch."""
+    from lbm_amd import scenario, workloads
+    specs = workloads.cavity_set(n=n)
+    doms = []
+    for xl, yl, zl, tau, _ in specs:
+        d = capi.Domain(xl, yl, zl, tau, q=Q, storage="f32")
+        scenario.cavity(xl, yl, zl, lid=(workloads.U_LID, 0.0, 0.0)).apply(d)
+        d.finalize()
+        doms.append(d)
+    b = capi.Batch(doms)
+    b.tune(5)
+    info = b.info()
+    b.step(20)
+    b.sync()
+    t = b.time_steps(steps)
+    b.close()
+    for d in doms:
+        d.close()
+    return {"workload": f"{n} D3Q19 lid-driven cavities, edges log-uniform [16,96], Re U[10,1000], fp32",
+            "steps": steps, "cells": t["cells"], "ms_per_step": round(t["ms"] / steps, 5),
+            "mlups": round(t["mlups"], 1), "vec": info["vec"], "blocks_per_cu": info["blocks_per_cu"],
+            "tiles": info["ntiles"], "near_cells": info["nnear"]}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +180,8 @@ def main() -> None:
     ap.add_argument("--modes", default="f32,f16,bf16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="short copy-bandwidth run (3 copies) for rocprofv3 --pmc passes")
     a = ap.parse_args()
@@ -164,6 +232,8 @@ def main() -> None:
     }
     if "f16" in res and "f32" in res:
         line["f16_over_f32"] = round(res["f16"]["mlups"] / res["f32"]["mlups"], 3)
+    line["adjoint"] = None if a.no_adjoint else adjoint_leg(capi)
+    line["batch"] = None if a.no_batch else batch_leg(capi)
     line["cpu_baseline"] = None if a.no_cpu else cpu_baseline(rho, u, a.cpu_steps)
     print(json.dumps(line), flush=True)
 

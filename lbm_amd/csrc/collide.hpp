@@ -43,7 +43,7 @@ __device__ __forceinline__ void moments_exact(const double (&f)[L::Q], double& r
 }
 
 // Velocity with a caller-given density (Outflow/Pressure pass rho_ref,
-// include/boundary.hpp:239,304).
+// include/boundary.hpp:144,209).
 template <class L>
 __device__ __forceinline__ void velocity_exact(const double (&f)[L::Q], double dens, double (&u)[3]) {
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
@@ -98,6 +98,65 @@ __device__ __forceinline__ void collide_exact(double (&f)[L::Q], double tau) {
 }
 
 // ---------------------------------------------------------------------------
+// Coefficient 3 w_q of the odd (momentum-carrying) part of the equilibrium.
+//
+// In fp32 the constant for each velocity class (|c|^2 = 1, 2, 3) is rounded
+// to the side that keeps the first-moment sum rule sum_q c_qx^2 (3 w_q) = 1
+// exact in real arithmetic. Rounded to nearest, the D3Q15/D3Q19 constants
+// (mantissa 4/3) all sit 2^-25 above their value and the sum is 1 + 3e-8, so
+// every collision would scale the momentum by 1 + omega * 3e-8: a velocity
+// drift of 2.4e-5 relative after 500 steps of a Taylor-Green box. The
+// rounding errors of the two directions are +2^-25 and -2^-24 (mantissa 4/3)
+// or +2^-27 and -2^-24 (mantissa 16/9), and the class shares of the sum are
+// D3Q15 2/3, 1/3; D3Q19 1/3, 2/3; D3Q27 4/9, 4/9, 1/9, which fixes the choice
+// below. fp64 keeps the nearest constant (its drift is 1e-16 per step).
+template <class L>
+struct OddCoef32;
+template <>
+struct OddCoef32<D3Q15> {  // axis 1/3 up, diagonal 1/24 down
+    static constexpr float c[4] = {0.0f, 0x1.555556p-2f, 0.0f, 0x1.555554p-5f};
+};
+template <>
+struct OddCoef32<D3Q19> {  // axis 1/6 down, diagonal 1/12 up
+    static constexpr float c[4] = {0.0f, 0x1.555554p-3f, 0x1.555556p-4f, 0.0f};
+};
+template <>
+struct OddCoef32<D3Q27> {  // axis 2/9 up, edge 1/18 up, corner 1/72 down
+    static constexpr float c[4] = {0.0f, 0x1.c71c72p-3f, 0x1.c71c72p-5f, 0x1.c71c70p-7f};
+};
+
+template <class L>
+LBM_HD constexpr int speed_class(int q) {
+    return L::cx(q) * L::cx(q) + L::cy(q) * L::cy(q) + L::cz(q) * L::cz(q);
+}
+
+template <class L, class Calc>
+LBM_HD constexpr Calc odd_coef(int q) {
+    if constexpr (std::is_same<Calc, float>::value)
+        return OddCoef32<L>::c[speed_class<L>(q)];
+    else
+        return static_cast<Calc>(3.0 * weight<L>(q));
+}
+
+// The fp32 constants are within one ulp of 3 w_q and satisfy the sum rule
+// exactly (the double sum of these few floats is exact).
+template <class L>
+constexpr bool odd_coef32_ok() {
+    double sum = 0;
+    for (int q = 0; q < L::Q; ++q) {
+        if (q == L::rest) continue;
+        const double exact = 3.0 * weight<L>(q);
+        const double v = static_cast<double>(odd_coef<L, float>(q));
+        const double d = v > exact ? v - exact : exact - v;
+        if (!(d < exact * 0x1p-23)) return false;
+        sum += static_cast<double>(L::cx(q) * L::cx(q)) * v;
+    }
+    return sum == 1.0;
+}
+static_assert(odd_coef32_ok<D3Q15>() && odd_coef32_ok<D3Q19>() && odd_coef32_ok<D3Q27>(),
+              "fp32 odd-part coefficients break the momentum sum rule");
+
+// ---------------------------------------------------------------------------
 // Shifted BGK: g_q = f_q - w_q, arithmetic in Calc (float or double),
 // 1/c_s^2 = 3 exactly. With a Brinkman field the equilibrium velocity is
 // (1 - alpha) * (u + tau * g) (porosity scaling of the equilibrium velocity
@@ -105,7 +164,12 @@ __device__ __forceinline__ void collide_exact(double (&f)[L::Q], double tau) {
 //   drho = sum g, rho = 1 + drho, u = sum c g / rho
 //   geq_q = w_q * (drho + rho * (3 cu + 4.5 cu^2 - 1.5 u.u))
 //   g_q  += omega * (geq_q - g_q)
-template <class L, class Calc, bool BRINK>
+// u is a true quotient: a reciprocal of rho multiplied into j rounds twice,
+// and over a flow's smooth density field those roundings bias the momentum
+// (5e-6 relative drift after 500 fp32 steps of a Taylor-Green box, against
+// 6e-7 with the division). RCP selects the cheaper reciprocal form, used
+// with 16-bit storage, whose own rounding (~1e-3) dwarfs that bias.
+template <class L, class Calc, bool BRINK, bool RCP = false>
 __device__ __forceinline__ void collide_shifted(Calc (&g)[L::Q], Calc omega, Calc alpha,
                                                 Calc tgx, Calc tgy, Calc tgz) {
     Calc drho = Calc(0), jx = Calc(0), jy = Calc(0), jz = Calc(0);
@@ -120,8 +184,17 @@ __device__ __forceinline__ void collide_shifted(Calc (&g)[L::Q], Calc omega, Cal
         if constexpr (L::cz(q) < 0) jz -= g[q];
     });
     const Calc rho = Calc(1) + drho;
-    const Calc inv_rho = Calc(1) / rho;
-    Calc ux = jx * inv_rho, uy = jy * inv_rho, uz = jz * inv_rho;
+    Calc ux, uy, uz;
+    if constexpr (RCP) {
+        const Calc inv_rho = Calc(1) / rho;
+        ux = jx * inv_rho;
+        uy = jy * inv_rho;
+        uz = jz * inv_rho;
+    } else {
+        ux = jx / rho;
+        uy = jy / rho;
+        uz = jz / rho;
+    }
     if constexpr (BRINK) {
         const Calc s = Calc(1) - alpha;
         ux = s * (ux + tgx);
@@ -143,7 +216,8 @@ __device__ __forceinline__ void collide_shifted(Calc (&g)[L::Q], Calc omega, Cal
         if constexpr (L::cz(q) > 0) cu += uz;
         if constexpr (L::cz(q) < 0) cu -= uz;
         const Calc even = w * (base + Calc(4.5) * rho * cu * cu);
-        const Calc odd = w * Calc(3) * rho * cu;
+        constexpr Calc w3 = odd_coef<L, Calc>(q);
+        const Calc odd = w3 * rho * cu;
         g[q] = one_m_omega * g[q] + omega * (even + odd);
         g[qb] = one_m_omega * g[qb] + omega * (even - odd);
     });

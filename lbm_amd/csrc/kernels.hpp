@@ -181,7 +181,7 @@ __device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int
         if constexpr (M == Mode::EXACT64) {
             collide_exact<L>(f[i], a.tau);
         } else {
-            collide_shifted<L, Calc, BRINK>(f[i], static_cast<Calc>(a.omega),
+            collide_shifted<L, Calc, BRINK, kRcp<M>>(f[i], static_cast<Calc>(a.omega),
                                             BRINK ? alpha[i] : Calc(0),
                                             static_cast<Calc>(a.tau * a.g[0]),
                                             static_cast<Calc>(a.tau * a.g[1]),
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(256) k_bulk(StepArgs a, int nbx, int nby) {
 template <class L>
 struct RulesExact {
     static constexpr int Q = L::Q;
-    // MovingWall, boundary.hpp:150-161
+    // MovingWall, boundary.hpp:55-65
     template <int q>
     __device__ static double moving_wall(const double (&own)[Q], const Cond& cd) {
         double density = 0;
@@ -242,7 +242,7 @@ struct RulesExact {
         constexpr double cs = kCS;
         return finv + 2.0 * weight<L>(q) * density * c_dot_u / (cs * cs);
     }
-    // Outflow boundary.hpp:235-243 and Pressure boundary.hpp:301-307
+    // Outflow boundary.hpp:140-147 and Pressure boundary.hpp:206-211
     template <int q>
     __device__ static double outflow(const double (&own)[Q], const Cond& cd) {
         double u[3];
@@ -318,14 +318,14 @@ __device__ __forceinline__ typename Io<M, L>::Calc pull_nonfluid(
     const T* __restrict__ src = static_cast<const T*>(a.src);
     stale = false;
     switch (cd.kind) {
-        case BC_NOSLIP:  // boundary.hpp:121-124
+        case BC_NOSLIP:  // boundary.hpp:26-28
             return own[Q - 1 - q];
         case BC_MOVINGWALL:
             if constexpr (M == Mode::EXACT64)
                 return RulesExact<L>::template moving_wall<q>(own, cd);
             else
                 return RulesShifted<L, Calc>::template moving_wall<q>(own, cd);
-        case BC_INFLOW:  // boundary.hpp:271-274, feq precomputed on the host
+        case BC_INFLOW:  // boundary.hpp:176-178, feq precomputed on the host
             return (M == Mode::EXACT64) ? static_cast<Calc>(cd.feq[q]) : static_cast<Calc>(cd.geq[q]);
         case BC_OUTFLOW:
         case BC_PRESSURE:
@@ -333,7 +333,7 @@ __device__ __forceinline__ typename Io<M, L>::Calc pull_nonfluid(
                 return RulesExact<L>::template outflow<q>(own, cd);
             else
                 return RulesShifted<L, Calc>::template outflow<q>(own, cd);
-        case BC_FREESLIP: {  // boundary.hpp:186-208 with b = (sx,sy,sz), b + c_q = x
+        case BC_FREESLIP: {  // boundary.hpp:91-110 with b = (sx,sy,sz), b + c_q = x
             const int cand[6][3] = {{x, sy, sz}, {sx, y, sz}, {sx, sy, z},
                                     {sx, y, z},  {x, sy, z},  {x, y, sz}};
             const int refl[6][3] = {{-cx, cy, cz},  {cx, -cy, cz},  {cx, cy, -cz},
@@ -397,12 +397,12 @@ __device__ __forceinline__ void near_cell(const StepArgs& a, int x, int y, int z
         Calc alpha = Calc(0);
         if (a.alpha) alpha = static_cast<const Calc*>(a.alpha)[e];
         if (a.alpha) {
-            collide_shifted<L, Calc, true>(f, static_cast<Calc>(a.omega), alpha,
+            collide_shifted<L, Calc, true, kRcp<M>>(f, static_cast<Calc>(a.omega), alpha,
                                            static_cast<Calc>(a.tau * a.g[0]),
                                            static_cast<Calc>(a.tau * a.g[1]),
                                            static_cast<Calc>(a.tau * a.g[2]));
         } else {
-            collide_shifted<L, Calc, false>(f, static_cast<Calc>(a.omega), Calc(0), Calc(0),
+            collide_shifted<L, Calc, false, kRcp<M>>(f, static_cast<Calc>(a.omega), Calc(0), Calc(0),
                                             Calc(0), Calc(0));
         }
     }

@@ -60,6 +60,10 @@ struct StorageOf<Mode::BF16> {
     static constexpr bool shifted = true;
 };
 
+// Velocity from momentum by reciprocal (16-bit storage) or by division.
+template <Mode M>
+constexpr bool kRcp = sizeof(typename StorageOf<M>::T) == 2;
+
 // Per-direction scale of the 16-bit formats: 2^8 times the power of two
 // nearest to 1/w_q. |g_q / w_q| stays below ~1 for stable flows, so stored
 // magnitudes sit near 2^8, far from the fp16 subnormal range (< 2^-14) and

@@ -8,6 +8,9 @@
 - cavity_set: lid-driven cavities with edge lengths log-uniform in [16, 96]
   per axis and Reynolds numbers uniform in [10, 1000], realised through the
   relaxation time tau = 3 U_lid L_x / Re + 1/2 with U_lid = 0.05.
+- porosity_pair: the inverse problem's porosity fields, uniform [0, 1] on a
+  coarse 32^3 grid (seed s for the control, s + 1 for the target), upsampled
+  trilinearly (periodic) to the fine grid.
 """
 from __future__ import annotations
 
@@ -73,3 +76,11 @@ def cavity_set(n=512, lmin=16, lmax=96, re_min=10.0, re_max=1000.0, seed=12345):
         tau = 3.0 * U_LID * xl / re + 0.5
         out.append((xl, yl, zl, tau, re))
     return out
+
+
+def porosity_pair(n=128, nc=32, seed=2024):
+    """(coarse control, fine control, fine target) porosity fields."""
+    from .autograd import upsample_np
+    ac = np.random.default_rng(seed).uniform(0.0, 1.0, (nc, nc, nc))
+    at = np.random.default_rng(seed + 1).uniform(0.0, 1.0, (nc, nc, nc))
+    return ac, upsample_np(ac, (n, n, n)), upsample_np(at, (n, n, n))

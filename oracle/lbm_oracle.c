@@ -379,10 +379,10 @@ static void boundary_collide(const ora_domain* d, double* field, int64_t x, int6
         const double* nb = field + IDX(d, nx, ny, nz) * Q;
         const int qi = Q - 1 - q; /* inv(q), include/model.h */
         switch (cond->kind) {
-            case ORA_NOSLIP: /* boundary.hpp:121-124 */
+            case ORA_NOSLIP: /* boundary.hpp:26-28 */
                 cell[q] = nb[qi];
                 break;
-            case ORA_MOVINGWALL: { /* boundary.hpp:150-161 */
+            case ORA_MOVINGWALL: { /* boundary.hpp:55-65 */
                 const double density = compute_density(d, nb);
                 double c_dot_u = 0;
                 for (int k = 0; k < 3; ++k) c_dot_u += d->c[q][k] * cond->vel[k];
@@ -390,7 +390,7 @@ static void boundary_collide(const ora_domain* d, double* field, int64_t x, int6
                 cell[q] = finv + 2.0 * d->w[q] * density * c_dot_u / (CS * CS);
                 break;
             }
-            case ORA_FREESLIP: { /* boundary.hpp:186-208 */
+            case ORA_FREESLIP: { /* boundary.hpp:91-110 */
                 const int64_t xs = wrap(x + dx, d->xl, d->periodic[0]);
                 const int64_t ys = wrap(y + dy, d->yl, d->periodic[1]);
                 const int64_t zs = wrap(z + dz, d->zl, d->periodic[2]);
@@ -408,15 +408,15 @@ static void boundary_collide(const ora_domain* d, double* field, int64_t x, int6
                 }
                 break;
             }
-            case ORA_OUTFLOW:    /* boundary.hpp:235-243 */
-            case ORA_PRESSURE: { /* boundary.hpp:301-307 */
+            case ORA_OUTFLOW:    /* boundary.hpp:140-147 */
+            case ORA_PRESSURE: { /* boundary.hpp:206-211 */
                 double v[3], feq[MAXQ];
                 compute_velocity(d, nb, cond->rho, v);
                 compute_feq(d, cond->rho, v, feq);
                 cell[q] = feq[q] + feq[qi] - nb[qi];
                 break;
             }
-            case ORA_INFLOW: { /* boundary.hpp:271-274 */
+            case ORA_INFLOW: { /* boundary.hpp:176-178 */
                 double feq[MAXQ];
                 compute_feq(d, cond->rho, cond->vel, feq);
                 cell[q] = feq[q];

@@ -25,12 +25,12 @@ extern "C" {
 
 /* Handler kinds; numbering matches the shipped C-ABI (include/lbm_hip.h). */
 enum {
-    ORA_NOSLIP = 1,     /* include/boundary.hpp:110-126 */
-    ORA_MOVINGWALL = 2, /* include/boundary.hpp:139-163 */
-    ORA_FREESLIP = 3,   /* include/boundary.hpp:175-210 */
-    ORA_OUTFLOW = 4,    /* include/boundary.hpp:224-245 */
-    ORA_INFLOW = 5,     /* include/boundary.hpp:260-276 */
-    ORA_PRESSURE = 6    /* include/boundary.hpp:290-309 */
+    ORA_NOSLIP = 1,     /* include/boundary.hpp:14-31 */
+    ORA_MOVINGWALL = 2, /* include/boundary.hpp:35-68 */
+    ORA_FREESLIP = 3,   /* include/boundary.hpp:72-115 */
+    ORA_OUTFLOW = 4,    /* include/boundary.hpp:119-150 */
+    ORA_INFLOW = 5,     /* include/boundary.hpp:154-181 */
+    ORA_PRESSURE = 6    /* include/boundary.hpp:185-214 */
 };
 
 typedef struct ora_domain ora_domain;

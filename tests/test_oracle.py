@@ -259,7 +259,7 @@ def test_last_wins_box_order():
 @pytest.mark.parametrize("rho_ref", [1.0, 1.2])
 def test_outflow_divides_by_reference_density(rho_ref):
     """Outflow value = feq_q + feq_qbar - f_qbar of the adjacent cell, with
-    u = momentum / rho_ref and feq at rho_ref (include/boundary.hpp:239-242)."""
+    u = momentum / rho_ref and feq at rho_ref (include/boundary.hpp:143-147)."""
     q = 19
     c, w = lat(q)
     xl, yl, zl = 4, 2, 2
