@@ -255,7 +255,7 @@ typedef struct lbm_info {
 int lbm_get_info(lbm_domain* d, lbm_info* info);
 
 /* Tuning: cells per thread (must be one the mode supports; 0 = default) and
- * threads along x per block (64, 128 or 256; 0 = automatic). */
+ * threads along x per block (a divisor of 256; 0 = automatic). */
 int lbm_set_tuning(lbm_domain* d, int vec, int block_x);
 
 #ifdef __cplusplus

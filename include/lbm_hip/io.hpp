@@ -111,8 +111,8 @@ public:
             std::cout << "Usage: lbm [file] additional-options...\n"
                       << "  --collision-model bgk --tau T --timesteps N --timesteps-per-plot P\n"
                       << "  --scenario-file F --omp-threads N --output-dir D\n"
-                      << "  --lattice D3Q15|D3Q19|D3Q27 --storage f64|f32|f16|bf16" << std::endl;
-            return;
+                      << "  --lattice D3Q15|D3Q19|D3Q27 --storage f64|f32|f16|bf16|f64s" << std::endl;
+            std::exit(1);  // as configuration.h:109-114
         }
         std::map<std::string, std::string> file;
         if (!positional.empty()) {
@@ -174,7 +174,8 @@ public:
         if (_storage == "f32") return LBM_STORAGE_F32;
         if (_storage == "f16") return LBM_STORAGE_F16;
         if (_storage == "bf16") return LBM_STORAGE_BF16;
-        throw std::logic_error("storage must be f64, f32, f16 or bf16");
+        if (_storage == "f64s") return LBM_STORAGE_F64S;
+        throw std::logic_error("storage must be f64, f32, f16, bf16 or f64s");
     }
 };
 
@@ -370,8 +371,7 @@ void parse_boundary(const XmlNode& b, Domain<M>& domain) {  // scenario.h:90-128
 // working directory, as in the reference.
 template <class M>
 Domain_ptr<M> parse_scenario_file(const std::string& filename, Config& cfg,
-                                  FluidCollision<M>& collision,
-                                  const DeviceOptions& opt = DeviceOptions()) {
+                                  FluidCollision<M>& collision, const DeviceOptions& opt) {
     std::ifstream in(filename);
     if (!in) throw std::logic_error("XML file \"" + filename + "\" could not be read properly!");
     std::stringstream ss;
@@ -407,6 +407,16 @@ Domain_ptr<M> parse_scenario_file(const std::string& filename, Config& cfg,
     for (const auto& b : xd->children)
         if (b.name == "boundary") parse_boundary(b, *domain);
     return domain;
+}
+
+// The reference's three-argument form: the storage mode comes from the
+// configuration's `storage` key (fp64 in the reference's arithmetic when unset).
+template <class M>
+Domain_ptr<M> parse_scenario_file(const std::string& filename, Config& cfg,
+                                  FluidCollision<M>& collision) {
+    DeviceOptions opt;
+    opt.storage = cfg.storage_id();
+    return parse_scenario_file<M>(filename, cfg, collision, opt);
 }
 
 // ---------------------------------------------------------------- output --

@@ -7,11 +7,14 @@
 //   domain->set_nonfluid_cells_nullcollide();                    // domain.hpp:101-113
 //   for (t = 1..T) { domain->stream(); domain->swap(); domain->collide(); }
 //
-// On the GPU one time step is one fused kernel pass, so stream() advances
-// the domain by one step, swap() flips nothing the caller can see and
-// collide() is empty. Work is asynchronous: code that times a step must call
-// domain->sync() before reading the clock. Errors are thrown as
-// std::runtime_error, which the reference driver's catch block reports.
+// On the GPU one time step is one fused kernel pass, so stream() enqueues
+// the step, swap() flips nothing the caller can see and collide() waits for
+// the step to finish, so that a clock read after collide() (src/main.cpp:53)
+// times the step. step(n) enqueues n steps without waiting (sync() waits).
+// Errors are thrown as std::runtime_error, which the reference driver's catch
+// block reports. The headers under include/lbm_hip/compat carry the
+// reference's header names, so src/main.cpp compiles against this file
+// unchanged.
 #pragma once
 
 #include <array>
@@ -304,7 +307,7 @@ public:
         snap_valid_ = false;
     }
     void swap() {}
-    void collide() {}
+    void collide() { check(lbm_sync(h_)); }
     void step(uint64_t n) {
         ensure_final();
         check(lbm_step(h_, n));

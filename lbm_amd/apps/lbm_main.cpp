@@ -41,8 +41,7 @@ int run(lbm::io::Config& cfg) {
         const double start = now();
         domain->stream();
         domain->swap();
-        domain->collide();
-        domain->sync();  // the step is asynchronous on the device
+        domain->collide();  // returns when the step has finished on the device
         mlups_duration += now() - start;
         if (cfg.timesteps_per_plot() && t % cfg.timesteps_per_plot() == 0)
             lbm::io::write_vtk_file(*domain, cfg.output_dir(), cfg.output_filename(), t);

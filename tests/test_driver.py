@@ -124,3 +124,28 @@ def test_driver_vtk_mask_scenario(tmp_path):
     ro, uo, fluid = oracle_moments(sc, steps)
     assert fluid.sum() < fluid.size  # the mask made solids
     assert np.array_equal(rho[fluid], ro[fluid]) and np.array_equal(u[fluid], uo[fluid])
+
+
+REF_MAIN = Path("/root/reference/src/main.cpp")
+
+
+@pytest.mark.skipif(not REF_MAIN.exists(), reason="reference sources not present on this machine")
+@pytest.mark.parametrize("lattice", [15, 19, 27])
+def test_reference_driver_compiles_against_compat_headers(tmp_path, lattice):
+    """The reference's own driver (src/main.cpp), unmodified, compiles and links
+    against include/lbm_hip/compat (headers with the reference's names) and
+    liblbm_hip.so, for each lattice it selects with -DD3Q; its command-line and
+    error paths then behave as the reference's (usage, "An error occured: ...",
+    EXIT_FAILURE) before any device work."""
+    exe = tmp_path / "lbm_ref_driver"
+    cmd = ["g++", "-std=c++17", "-O1", "-fopenmp", f"-DD3Q={lattice}",
+           f"-I{ROOT / 'include' / 'lbm_hip' / 'compat'}", f"-I{ROOT / 'include'}", str(REF_MAIN),
+           f"-L{ROOT / 'lbm_amd'}", "-llbm_hip", f"-Wl,-rpath,{ROOT / 'lbm_amd'}", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    h = subprocess.run([str(exe), "--help"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert h.returncode == 1 and "Usage" in h.stdout  # the reference exits with 1 after the usage text
+    cfg = write_cfg(tmp_path, tmp_path / "missing.xml", 5, 0)
+    e = subprocess.run([str(exe), str(cfg)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert e.returncode == 1
+    assert "An error occured: XML file" in e.stderr

@@ -175,6 +175,37 @@ def test_couette_mixed_periodic_bit_exact(q):
     assert_pdf_identical(g, o, "couette")
 
 
+@pytest.mark.parametrize("storage,tol", [("f64s", 1e-10), ("f32", 2e-5)])
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_brinkman_channel_against_oracle(q, storage, tol):
+    """Porous channel: random porosity in [0, 1] per cell, body force, no-slip
+    walls in y and z, periodic in x, 300 steps from rest, against the
+    oracle's Brinkman extension (fp64, reference operation order)."""
+    xl, yl, zl = 12, 9, 7
+    rng = np.random.default_rng(31 + q)
+    alpha = rng.uniform(0.0, 1.0, xl * yl * zl)
+    g3 = (2e-5, 5e-6, 0.0)
+    sc = scenario.Scenario("porous", xl, yl, zl)
+    for ext in ("y0", "ymax", "z0", "zmax"):
+        sc.boundaries.append(scenario.Boundary(1, (0, 0, 0, 1.0), scenario.face_box(ext, xl, yl, zl)))
+    say(f"brinkman channel q={q} {storage}")
+    g = capi.Domain(xl, yl, zl, 0.7, q=q, storage=storage, periodic=(1, 0, 0))
+    o = ora.OracleDomain(xl, yl, zl, 0.7, q=q, periodic=(1, 0, 0))
+    for d in (g, o):
+        sc.apply(d)
+        d.set_brinkman(alpha, g3)
+        d.finalize()
+    g.step(300)
+    o.step(300)
+    rg, ug = g.moments()
+    ro, uo = o.moments()
+    eu = rel_err(ug, uo)
+    er = rel_err(rg - 1, ro - 1)
+    say(f"  u rel L2 {eu[0]:.2e} max {eu[1]:.2e}; drho rel L2 {er[0]:.2e}; max |u| {np.abs(uo).max():.2e}")
+    assert np.abs(uo).max() > 1e-5  # the force drives a flow
+    assert eu[0] < tol and eu[1] < 10 * tol
+
+
 def test_set_pdf_midrun_bit_exact():
     n = 10
     say("set_pdf mid-run")

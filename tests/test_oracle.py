@@ -7,7 +7,10 @@ momentum and relaxes the non-equilibrium part by (1 - 1/tau), the textbook
 equilibrium is a fixed point, half-way bounce-back conserves mass, Couette
 flow is linear between a no-slip and a moving wall, uniform equilibrium flow
 passes inflow/outflow/pressure/free-slip boundaries unchanged, and the
-first-step and last-wins semantics of the reference's object model.
+first-step and last-wins semantics of the reference's object model. The
+extensions (equilibrium initialisation, Brinkman porosity with a body force)
+are pinned by their closed forms: a uniform porous box follows the scalar
+recurrence u' = u (1 - omega alpha) + omega (1 - alpha) tau g exactly.
 """
 import numpy as np
 import pytest
@@ -290,3 +293,49 @@ def test_unassigned_ghost_rejected():
         d.set_boundary_box("noslip", None, *scenario.face_box(ext, 4, 4, 4))
     with pytest.raises(ValueError):
         d.finalize()
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_init_equilibrium_is_textbook_feq(q):
+    """f = feq(rho, u) per cell (include/collision.hpp:34-51 with the
+    reference's C_S literal), so the moments return rho and u."""
+    c, w = lat(q)
+    rng = np.random.default_rng(5)
+    n = 3
+    rho = 1 + 0.01 * rng.standard_normal((n, n, n))
+    u = 0.03 * rng.standard_normal((n, n, n, 3))
+    d = ora.OracleDomain(n, n, n, 0.7, q=q, periodic=(1, 1, 1))
+    d.finalize()
+    d.init_equilibrium(rho, u)
+    f = interior(d.get_pdf())
+    assert np.allclose(f, feq_textbook(c, w, rho, u), rtol=0, atol=1e-13)
+    r1, u1 = d.moments()
+    assert np.allclose(r1, rho, rtol=1e-12, atol=0)
+    assert np.allclose(u1, u, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+@pytest.mark.parametrize("alpha", [0.0, 0.3, 1.0])
+def test_brinkman_uniform_box_recurrence(q, alpha):
+    """Uniform porosity and body force on a periodic box at rest: every cell
+    stays uniform and u_n obeys u' = u (1 - omega alpha) + omega (1 - alpha)
+    tau g, whose fixed point is the Darcy plateau (1 - alpha) tau g / alpha;
+    alpha = 0 accelerates freely (u_n = n g) and alpha = 1 only decays."""
+    tau, g, n, steps = 0.9, np.array([2e-5, -1e-5, 5e-6]), 2, 60
+    d = ora.OracleDomain(n, n, n, tau, q=q, periodic=(1, 1, 1))
+    d.set_brinkman(np.full(n ** 3, alpha), g)
+    d.finalize()
+    d.init_equilibrium(np.ones((n, n, n)), np.zeros((n, n, n, 3)))
+    om = 1 / tau
+    u = np.zeros(3)
+    for _ in range(steps):
+        u = u * (1 - om * alpha) + om * (1 - alpha) * tau * g
+    d.step(steps)
+    rho1, u1 = d.moments()
+    assert np.allclose(u1, u, rtol=1e-9, atol=1e-15)
+    assert np.allclose(rho1, 1.0, rtol=0, atol=1e-13)
+    if alpha == 0.0:
+        assert np.allclose(u1, steps * g, rtol=1e-9)
+    if 0 < alpha < 1:
+        plateau = (1 - alpha) * tau * g / alpha
+        assert np.all(np.abs(u) < np.abs(plateau))
