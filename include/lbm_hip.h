@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define LBM_HIP_ABI_VERSION 1
+#define LBM_HIP_ABI_VERSION 2
 
 /* Status codes. */
 #define LBM_OK 0
@@ -206,8 +206,10 @@ int lbm_adj_timing(lbm_tape* t, double out[4]);
 /* ---------------------------------------------------------------------------
  * Batching (extension; the reference runs one domain per process). A batch
  * steps many finalized domains of any sizes and relaxation times together:
- * one persistent bulk launch walks a tile work list over all domains and one
- * launch covers all near-boundary cells; lbm_batch_step captures the whole
+ * one persistent launch walks a tile work list over all domains and steps
+ * their bulk and near-boundary cells (or, as the tuner decides, a second
+ * launch covers the near-boundary cells; see lbm_batch_kernel);
+ * lbm_batch_step captures the whole
  * run of nsteps as a hipGraph and launches it once. Results are bit-identical
  * to stepping each domain alone. All domains share the lattice and storage
  * (D3Q19; F32, F16, BF16 or F64S), the device, and their step parity; the
@@ -219,7 +221,8 @@ typedef struct lbm_batch lbm_batch;
 int lbm_batch_create(lbm_domain* const* domains, int n, lbm_batch** out);
 int lbm_batch_destroy(lbm_batch* b);
 /* Autotuner: times `trial_steps` steps of every candidate launch shape
- * (cells per thread, threads along x per tile, persistent blocks per CU) and
+ * (kernel form, cells per thread, threads along x per tile, persistent
+ * blocks per CU) and
  * keeps the fastest; best_ms receives its time per step. */
 int lbm_batch_tune(lbm_batch* b, int trial_steps, double* best_ms);
 int lbm_batch_step(lbm_batch* b, uint64_t nsteps);
@@ -237,8 +240,24 @@ typedef struct lbm_batch_info {
     int blocks_per_cu; /* persistent blocks per CU (0: one block per tile) */
     int bx_policy;     /* 0: smallest power of two covering a row, 1: half of it */
     int ncu;
+    int kernel;        /* LBM_BATCH_ROWS_SEPARATE, LBM_BATCH_ROWS_FUSED or LBM_BATCH_CELLS */
 } lbm_batch_info;
 int lbm_batch_get_info(lbm_batch* b, lbm_batch_info* info);
+/* Kernel forms of a batched step. ROWS_*: blocks of threads over (x, y) rows
+ * of one z plane, V cells per thread, near-boundary cells in a second launch
+ * over their list (SEPARATE) or in the same pass (FUSED). CELLS: 64
+ * consecutive cells of a z plane per wavefront, one cell per lane, every cell
+ * kind in one pass. */
+enum lbm_batch_kernel {
+    LBM_BATCH_ROWS_SEPARATE = 0,
+    LBM_BATCH_ROWS_FUSED = 1,
+    LBM_BATCH_CELLS = 2
+};
+/* Sets a launch configuration directly (what lbm_batch_tune chooses):
+ * kernel form, cells per thread (ROWS_*: one the storage mode supports),
+ * bx_policy 0 or 1 (ROWS_*), persistent blocks per CU (0: one block per
+ * tile or per four cell tiles). */
+int lbm_batch_set_config(lbm_batch* b, int kernel, int vec, int bx_policy, int blocks_per_cu);
 
 /* Layout and size information. */
 typedef struct lbm_info {
@@ -251,11 +270,16 @@ typedef struct lbm_info {
     int vec;                   /* cells per thread of the bulk kernel */
     int block_x, block_y;      /* bulk kernel block shape */
     int nconds;                /* distinct boundary conditions */
+    int cell_tiles;            /* 1: steps run as 64-cell tiles per wavefront (k_cells) */
 } lbm_info;
 int lbm_get_info(lbm_domain* d, lbm_info* info);
 
-/* Tuning: cells per thread (must be one the mode supports; 0 = default) and
- * threads along x per block (a divisor of 256; 0 = automatic). */
+/* Tuning. vec = 0 (default): domains with boundary cells step as 64-cell
+ * tiles per wavefront, every cell kind in one kernel; bulk-only domains use
+ * the row kernel with the mode's default cells per thread. vec > 0: the row
+ * kernel with vec cells per thread (one the mode supports) plus a kernel over
+ * the near-boundary cells. block_x: threads along x per row-kernel block (a
+ * divisor of 256; 0 = automatic). Results are the same bits either way. */
 int lbm_set_tuning(lbm_domain* d, int vec, int block_x);
 
 #ifdef __cplusplus

@@ -45,7 +45,7 @@ class Info(ctypes.Structure):
     _fields_ = [("ncells_interior", c_uint64), ("ncells_fluid", c_uint64),
                 ("ncells_near", c_uint64), ("bytes_per_value", c_uint64),
                 ("field_bytes", c_uint64), ("device_bytes", c_uint64), ("vec", c_int),
-                ("block_x", c_int), ("block_y", c_int), ("nconds", c_int)]
+                ("block_x", c_int), ("block_y", c_int), ("nconds", c_int), ("cell_tiles", c_int)]
 
 
 # (name, restype, argtypes) of every exported entry point in include/lbm_hip.h
@@ -83,6 +83,7 @@ SIGNATURES = [
     ("lbm_batch_sync", c_int, [c_void_p]),
     ("lbm_batch_time_steps", c_int, [c_void_p, c_uint64, POINTER(c_double)]),
     ("lbm_batch_get_info", c_int, [c_void_p, c_void_p]),
+    ("lbm_batch_set_config", c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     ("lbm_get_info", c_int, [c_void_p, POINTER(Info)]),
     ("lbm_set_tuning", c_int, [c_void_p, c_int, c_int]),
 ]
@@ -324,10 +325,13 @@ class Tape:
         return {"recompute_ms": out[0], "adjoint_ms": out[1], "forward_ms": out[2], "steps": int(out[3])}
 
 
+BATCH_ROWS_SEPARATE, BATCH_ROWS_FUSED, BATCH_CELLS = 0, 1, 2
+
+
 class BatchInfo(ctypes.Structure):
     _fields_ = [("ndomains", c_uint64), ("ncells", c_uint64), ("ntiles", c_uint64),
                 ("nnear", c_uint64), ("vec", c_int), ("blocks_per_cu", c_int),
-                ("bx_policy", c_int), ("ncu", c_int)]
+                ("bx_policy", c_int), ("ncu", c_int), ("kernel", c_int)]
 
 
 class Batch:
@@ -367,6 +371,10 @@ class Batch:
         out = np.zeros(2)
         _check(load().lbm_batch_time_steps(self._h, n, _dptr(out)))
         return {"ms": float(out[0]), "cells": int(out[1]), "mlups": out[1] * n / (out[0] * 1e3)}
+
+    def set_config(self, kernel: int = 2, vec: int = 1, bx_policy: int = 0, blocks_per_cu: int = 8) -> None:
+        """kernel: BATCH_ROWS_SEPARATE, BATCH_ROWS_FUSED or BATCH_CELLS."""
+        _check(load().lbm_batch_set_config(self._h, kernel, vec, bx_policy, blocks_per_cu))
 
     def info(self) -> dict:
         i = BatchInfo()

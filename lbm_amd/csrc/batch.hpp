@@ -21,14 +21,26 @@ struct BatchView {
     const uint8_t* kind;
     const Cond* conds;
     double tau, omega;
-    int bx_log2;  // threads along x per tile = 1 << bx_log2 (of 256)
+    int bx_log2;  // row tiles: threads along x per tile = 1 << bx_log2 (of 256)
     int nbx, nby;
-    int pad;
+    int tpp;      // cell tiles: 64-cell tiles per z plane = ceil(xl * yl / 64)
+    float rxl;    // 1 / xl (cell tiles: row of a plane index)
 };
 
+// Row tile: a block of BX x BY threads over (x, y) at one z, local =
+// bx + nbx * (by + nby * (z - 1)). Cell tile: 64 consecutive interior cells
+// (x fastest) of one z plane for one wavefront, local = i + tpp * (z - 1),
+// cells 64 i .. 64 i + 63 of the plane.
 struct TileRef {
     int dom;
-    int local;  // bx + nbx * (by + nby * (z - 1))
+    int local;
+};
+
+// Kernel forms of a batched step.
+enum BatchKernel : int {
+    BK_ROWS_SEPARATE = 0,  // row tiles for bulk cells + a launch over the NEAR list
+    BK_ROWS_FUSED = 1,     // row tiles stepping their NEAR cells too
+    BK_CELLS = 2,          // 64-cell tiles per wavefront, every cell kind in one pass
 };
 
 struct NearRef {
@@ -50,7 +62,7 @@ struct BatchOps {
     int q;
     Mode mode;
     int vecs[4];
-    hipError_t (*launch)(const BatchArgs& a, int V, int grid_bulk, int grid_near, hipStream_t s);
+    hipError_t (*launch)(const BatchArgs& a, int V, int kernel, int grid_bulk, int grid_near, hipStream_t s);
 };
 
 const BatchOps* batch_ops(int q, Mode m);

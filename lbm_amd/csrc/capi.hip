@@ -92,6 +92,14 @@ int make_plan(lbm_domain* d) {
     p.nby = (d->L.yl + p.BY - 1) / p.BY;
     p.nblocks_bulk = static_cast<int64_t>(p.nbx) * p.nby * d->L.zl;
     if (p.nblocks_bulk > 0x7fffffffLL) return fail(LBM_ERR_UNSUPPORTED, "grid too large");
+    // Domains with boundary cells step in cell tiles unless a vector width was
+    // asked for; bulk-only (periodic) domains keep the vectorised row kernel.
+    const int64_t plane_cells = static_cast<int64_t>(d->L.xl) * d->L.yl;
+    p.tpp = static_cast<int>((plane_cells + 63) / 64);
+    p.nblocks_cells = (static_cast<int64_t>(p.tpp) * d->L.zl + 3) / 4;
+    p.rxl = 1.0 / static_cast<double>(d->L.xl);
+    p.cells = d->tune_vec == 0 && d->nb > 0 && p.nblocks_cells <= 0x7fffffffLL &&
+              (plane_cells + 63) / 64 <= 0x7fffffffLL;
     return LBM_OK;
 }
 
@@ -639,6 +647,7 @@ int lbm_get_info(lbm_domain* d, lbm_info* info) {
     info->block_x = d->plan.BX;
     info->block_y = d->plan.BY;
     info->nconds = d->nconds_dev;
+    info->cell_tiles = d->plan.cells ? 1 : 0;
     return LBM_OK;
 }
 

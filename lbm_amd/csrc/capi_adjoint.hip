@@ -55,6 +55,7 @@ void tape_free(lbm_tape* t) {
 
 StepPlan plan_v1(const lbm_domain* d) {
     StepPlan p = d->plan;
+    p.cells = false;  // the adjoint and dual steps run the row kernels
     p.V = 1;
     int bx = 1;
     while (bx < d->L.xl && bx < 256) bx *= 2;

@@ -26,6 +26,8 @@ struct lbm_batch {
     int ncu = 0;
     hipStream_t stream = nullptr;
     int V = 0, bpc = 8, bx_policy = 0;
+    int kernel = BK_CELLS;      // enum BatchKernel
+    int64_t near_cells = 0;     // NEAR cells of all domains
     BatchView* d_views = nullptr;
     TileRef* d_tiles = nullptr;
     NearRef* d_near = nullptr;
@@ -56,8 +58,12 @@ void free_lists(lbm_batch* b) {
     b->d_near = nullptr;
 }
 
-// Builds views, tiles and the near list for (V, bx_policy) and uploads them.
-int configure(lbm_batch* b, int V, int bx_policy, int bpc) {
+// Builds views, tiles and (for BK_ROWS_SEPARATE) the near list and uploads
+// them. Row tiles depend on (V, bx_policy); with the NEAR cells fused into
+// the tile kernel every domain gets tiles over all its rows. Cell tiles cover
+// every plane of every domain in 64-cell pieces.
+int configure(lbm_batch* b, int V, int bx_policy, int bpc, int kernel) {
+    if (kernel == BK_CELLS) V = 1;
     drop_graph(b);
     std::vector<BatchView> views;
     std::vector<TileRef> tiles;
@@ -85,15 +91,23 @@ int configure(lbm_batch* b, int V, int bx_policy, int bpc) {
         v.bx_log2 = l2;
         v.nbx = (nvx + bx - 1) / bx;
         v.nby = (d->L.yl + (256 / bx) - 1) / (256 / bx);
-        const int64_t nt = static_cast<int64_t>(v.nbx) * v.nby * d->L.zl;
-        if (d->plan.has_bulk)
+        v.tpp = static_cast<int>((static_cast<int64_t>(d->L.xl) * d->L.yl + 63) / 64);
+        v.rxl = 1.0f / static_cast<float>(d->L.xl);
+        const int64_t nt = kernel == BK_CELLS ? static_cast<int64_t>(v.tpp) * d->L.zl
+                                              : static_cast<int64_t>(v.nbx) * v.nby * d->L.zl;
+        // cell tiles split a plane index with a float reciprocal (exact below 2^22)
+        if (nt > 0x3fffffff || static_cast<int64_t>(d->L.xl) * d->L.yl > (int64_t{1} << 22))
+            return fail(LBM_ERR_UNSUPPORTED, "domain too large for a batch tile index");
+        if (kernel != BK_ROWS_SEPARATE || d->plan.has_bulk)
             for (int64_t t = 0; t < nt; ++t) tiles.push_back(TileRef{static_cast<int>(di), static_cast<int>(t)});
         views.push_back(v);
     }
     // near cells: the domains' own lists, gathered
+    b->near_cells = 0;
     for (size_t di = 0; di < b->doms.size(); ++di) {
         const lbm_domain* d = b->doms[di];
-        if (!d->nb) continue;
+        b->near_cells += d->nb;
+        if (!d->nb || kernel != BK_ROWS_SEPARATE) continue;
         std::vector<CellCoord> bl(static_cast<size_t>(d->nb));
         HIPCHK(hipMemcpy(bl.data(), d->d_blist, bl.size() * sizeof(CellCoord), hipMemcpyDeviceToHost));
         for (const CellCoord& c : bl) near.push_back(NearRef{static_cast<int>(di), c.x, c.y, c.z});
@@ -109,11 +123,13 @@ int configure(lbm_batch* b, int V, int bx_policy, int bpc) {
         HIPCHK(hipMalloc(&b->d_near, near.size() * sizeof(NearRef)));
         HIPCHK(hipMemcpy(b->d_near, near.data(), near.size() * sizeof(NearRef), hipMemcpyHostToDevice));
     }
+    if (tiles.size() > 0x3fffffffu) return fail(LBM_ERR_UNSUPPORTED, "too many batch tiles");
     b->ntiles = static_cast<int64_t>(tiles.size());
     b->nnear = static_cast<int64_t>(near.size());
     b->V = V;
     b->bx_policy = bx_policy;
     b->bpc = bpc;
+    b->kernel = kernel;
     return LBM_OK;
 }
 
@@ -134,15 +150,16 @@ hipError_t enqueue(lbm_batch* b, uint64_t nsteps, int parity, int mat) {
     a.ntiles = b->ntiles;
     a.near = b->d_near;
     a.nnear = b->nnear;
-    const int64_t full = std::max<int64_t>(1, b->ntiles);
+    // blocks that give every tile its own block (cell tiles: 4 wavefronts a block)
+    const int64_t full = std::max<int64_t>(1, b->kernel == BK_CELLS ? (b->ntiles + 3) / 4 : b->ntiles);
     const int grid_bulk = static_cast<int>(b->bpc ? std::min<int64_t>(full, static_cast<int64_t>(b->ncu) * b->bpc)
                                                   : std::min<int64_t>(full, 0x7fffffff));
-    const int grid_near = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((b->nnear + 127) / 128,
-                                                                                  static_cast<int64_t>(b->ncu) * 16)));
+    const int grid_near = static_cast<int>(
+        std::max<int64_t>(1, std::min<int64_t>((b->nnear + 127) / 128, static_cast<int64_t>(b->ncu) * 16)));
     for (uint64_t s = 0; s < nsteps; ++s) {
         a.parity = parity;
         a.materialised = (s == 0) ? mat : 0;
-        const hipError_t e = b->ops->launch(a, b->V, grid_bulk, grid_near, b->stream);
+        const hipError_t e = b->ops->launch(a, b->V, b->kernel, grid_bulk, grid_near, b->stream);
         if (e != hipSuccess) return e;
         parity ^= 1;
     }
@@ -196,7 +213,7 @@ int lbm_batch_create(lbm_domain* const* domains, int n, lbm_batch** out) {
     int V0 = ops->vecs[0];
     for (int i = 0; i < 4 && ops->vecs[i]; ++i)
         if (ops->vecs[i] == d0->plan.V) V0 = d0->plan.V;
-    int rc = configure(b, V0, 0, 8);
+    int rc = configure(b, V0, 0, 8, BK_CELLS);
     if (rc) {
         free_lists(b);
         (void)hipStreamDestroy(b->stream);
@@ -299,7 +316,7 @@ int lbm_batch_tune(lbm_batch* b, int trial_steps, double* best_ms) {
     if (int rc = check_parity(b, parity, mat)) return rc;
     if (int rc = sync_domains(b)) return rc;
     double best = 1e300;
-    int bv = b->V, bp = b->bx_policy, bb = b->bpc;
+    int bv = b->V, bp = b->bx_policy, bb = b->bpc, bk = b->kernel;
     // The trials step the real fields; save every domain's current field and
     // restore it afterwards, so tuning leaves the state as it found it.
     // (Trial steps write only fluid cells, which the next step overwrites in
@@ -326,31 +343,36 @@ int lbm_batch_tune(lbm_batch* b, int trial_steps, double* best_ms) {
     // Kernels are compiled without FMA contraction, so a cell's arithmetic is
     // the same for every V, tile shape and grid size: the search changes
     // speed only, never results.
-    for (int i = 0; i < 4 && b->ops->vecs[i]; ++i)
-        for (int pol = 0; pol < 2; ++pol)
-            for (int bpc : {0, 4, 8, 16}) {
-                if (int rc = configure(b, b->ops->vecs[i], pol, bpc)) return rc;
-                // trial runs advance the fields; tuning happens before the timed run
-                int p, m;
-                check_parity(b, p, m);
-                HIPCHK(enqueue(b, 1, p, m));  // warm-up
-                advance_domains(b, 1);
-                check_parity(b, p, m);
-                HIPCHK(hipEventRecord(e0, b->stream));
-                HIPCHK(enqueue(b, static_cast<uint64_t>(trial_steps), p, m));
-                HIPCHK(hipEventRecord(e1, b->stream));
-                HIPCHK(hipEventSynchronize(e1));
-                advance_domains(b, static_cast<uint64_t>(trial_steps));
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                const double per = ms / trial_steps;
-                if (per < best) {
-                    best = per;
-                    bv = b->V;
-                    bp = pol;
-                    bb = bpc;
+    for (int kern : {BK_CELLS, BK_ROWS_FUSED, BK_ROWS_SEPARATE}) {
+        for (int i = 0; i < 4 && b->ops->vecs[i]; ++i)
+            for (int pol = 0; pol < 2; ++pol)
+                for (int bpc : {0, 4, 8, 16}) {
+                    // cell tiles have one shape: try it once per grid size
+                    if (kern == BK_CELLS && (i > 0 || pol > 0)) continue;
+                    if (int rc = configure(b, b->ops->vecs[i], pol, bpc, kern)) return rc;
+                    // trial runs advance the fields; tuning happens before the timed run
+                    int p, m;
+                    check_parity(b, p, m);
+                    HIPCHK(enqueue(b, 1, p, m));  // warm-up
+                    advance_domains(b, 1);
+                    check_parity(b, p, m);
+                    HIPCHK(hipEventRecord(e0, b->stream));
+                    HIPCHK(enqueue(b, static_cast<uint64_t>(trial_steps), p, m));
+                    HIPCHK(hipEventRecord(e1, b->stream));
+                    HIPCHK(hipEventSynchronize(e1));
+                    advance_domains(b, static_cast<uint64_t>(trial_steps));
+                    float ms = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    const double per = ms / trial_steps;
+                    if (per < best) {
+                        best = per;
+                        bv = b->V;
+                        bp = pol;
+                        bb = bpc;
+                        bk = kern;
+                    }
                 }
-            }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     for (size_t i = 0; i < b->doms.size(); ++i) {
@@ -361,9 +383,23 @@ int lbm_batch_tune(lbm_batch* b, int trial_steps, double* best_ms) {
     }
     HIPCHK(hipStreamSynchronize(b->stream));
     release();
-    if (int rc = configure(b, bv, bp, bb)) return rc;
+    if (int rc = configure(b, bv, bp, bb, bk)) return rc;
     if (best_ms) *best_ms = best;
     return LBM_OK;
+}
+
+int lbm_batch_set_config(lbm_batch* b, int kernel, int vec, int bx_policy, int blocks_per_cu) {
+    if (!b) return fail(LBM_ERR_ARG, "null batch");
+    bool ok = false;
+    for (int i = 0; i < 4 && b->ops->vecs[i]; ++i) ok |= (b->ops->vecs[i] == vec);
+    if (!ok && kernel != BK_CELLS) return fail(LBM_ERR_ARG, "cells per thread not built for this storage mode");
+    if (bx_policy < 0 || bx_policy > 1 || blocks_per_cu < 0 || blocks_per_cu > 64 || kernel < BK_ROWS_SEPARATE ||
+        kernel > BK_CELLS)
+        return fail(LBM_ERR_ARG, "bad batch configuration");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    drop_graph(b);
+    return configure(b, vec, bx_policy, blocks_per_cu, kernel);
 }
 
 int lbm_batch_get_info(lbm_batch* b, lbm_batch_info* info) {
@@ -371,11 +407,12 @@ int lbm_batch_get_info(lbm_batch* b, lbm_batch_info* info) {
     info->ndomains = b->doms.size();
     info->ncells = b->ncells;
     info->ntiles = static_cast<uint64_t>(b->ntiles);
-    info->nnear = static_cast<uint64_t>(b->nnear);
+    info->nnear = static_cast<uint64_t>(b->near_cells);
     info->vec = b->V;
     info->blocks_per_cu = b->bpc;
     info->bx_policy = b->bx_policy;
     info->ncu = b->ncu;
+    info->kernel = b->kernel;
     return LBM_OK;
 }
 

@@ -97,7 +97,8 @@ __device__ __forceinline__ int wrap_if(int v, int l, int per) {
 // from a ghost cell, is NEAR and is masked out before its values are used.
 // V cells (x0 .. x0+V-1, y, z) of one thread.
 template <class L, Mode M, int V, bool FLAGS, bool BRINK>
-__device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int z) {
+__device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int z,
+                                           bool* near_lanes = nullptr) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
@@ -114,6 +115,7 @@ __device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int
         for (int i = 0; i < V; ++i) {
             act[i] = (kv.v[i] == KIND_BULK) && (x0 + i <= Ly.xl);
             any |= act[i];
+            if (near_lanes) near_lanes[i] = (kv.v[i] == KIND_NEAR) && (x0 + i <= Ly.xl);
         }
     } else {
 #pragma unroll
@@ -225,69 +227,100 @@ __global__ void __launch_bounds__(256) k_bulk(StepArgs a, int nbx, int nby) {
 // own[] holds x's post-collision values (the values the reference's rule
 // reads from its fluid neighbour), in the Calc representation of the mode.
 
-// Reference-order rule values (EXACT64), include/boundary.hpp.
-template <class L>
-struct RulesExact {
-    static constexpr int Q = L::Q;
-    // MovingWall, boundary.hpp:55-65
+// The rules read x's own values only through their plain sum, their three
+// momentum sums and the single value own[qbar]. OwnSums holds the sums, each
+// accumulated from 0 over ascending q: the order of the reference's density()
+// and velocity() (include/collision.hpp:7-31), which EXACT64 keeps bit for
+// bit (adding -v equals subtracting v; directions with c_d = 0 add nothing).
+template <class Calc>
+struct OwnSums {
+    Calc s, m0, m1, m2;
+};
+
+template <class L, class Calc>
+struct OwnAcc {
+    OwnSums<Calc> r{Calc(0), Calc(0), Calc(0), Calc(0)};
     template <int q>
-    __device__ static double moving_wall(const double (&own)[Q], const Cond& cd) {
-        double density = 0;
-        static_for<0, Q>([&](auto kc) { density += own[decltype(kc)::value]; });
-        double c_dot_u = 0;
-        c_dot_u += static_cast<double>(L::cx(q)) * cd.vel[0];
-        c_dot_u += static_cast<double>(L::cy(q)) * cd.vel[1];
-        c_dot_u += static_cast<double>(L::cz(q)) * cd.vel[2];
-        const double finv = own[Q - 1 - q];
-        constexpr double cs = kCS;
-        return finv + 2.0 * weight<L>(q) * density * c_dot_u / (cs * cs);
-    }
-    // Outflow boundary.hpp:140-147 and Pressure boundary.hpp:206-211
-    template <int q>
-    __device__ static double outflow(const double (&own)[Q], const Cond& cd) {
-        double u[3];
-        velocity_exact<L>(own, cd.rho, u);
-        const double fq = feq_exact<L, q>(cd.rho, u);
-        const double fb = feq_exact<L, Q - 1 - q>(cd.rho, u);
-        return fq + fb - own[Q - 1 - q];
+    __device__ __forceinline__ void add(Calc v) {
+        r.s += v;
+        if constexpr (L::cx(q) > 0) r.m0 += v;
+        if constexpr (L::cx(q) < 0) r.m0 -= v;
+        if constexpr (L::cy(q) > 0) r.m1 += v;
+        if constexpr (L::cy(q) < 0) r.m1 -= v;
+        if constexpr (L::cz(q) > 0) r.m2 += v;
+        if constexpr (L::cz(q) < 0) r.m2 -= v;
     }
 };
 
-// Shifted-value rule values (g = f - w), same rules.
 template <class L, class Calc>
-struct RulesShifted {
-    static constexpr int Q = L::Q;
-    template <int q>
-    __device__ static Calc moving_wall(const Calc (&own)[Q], const Cond& cd) {
-        Calc drho = Calc(0);
-        static_for<0, Q>([&](auto kc) { drho += own[decltype(kc)::value]; });
-        const Calc rho = Calc(1) + drho;
-        const Calc cu = static_cast<Calc>(L::cx(q) * cd.vel[0] + L::cy(q) * cd.vel[1] +
-                                          L::cz(q) * cd.vel[2]);
-        return own[Q - 1 - q] + static_cast<Calc>(6.0 * weight<L>(q)) * rho * cu;
+__device__ __forceinline__ OwnSums<Calc> own_sums(const Calc (&own)[L::Q]) {
+    OwnAcc<L, Calc> acc;
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        acc.template add<q>(own[q]);
+    });
+    return acc.r;
+}
+
+// Own values read from the source field at element e (one pass, nothing kept).
+template <class L, Mode M>
+__device__ __forceinline__ OwnSums<typename Io<M, L>::Calc> own_sums_at(
+    const typename Io<M, L>::T* __restrict__ src, int64_t plane, int64_t e) {
+    using IO = Io<M, L>;
+    OwnAcc<L, typename IO::Calc> acc;
+    static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        acc.template add<q>(IO::template to_calc<q>(src[q * plane + e]));
+    });
+    return acc.r;
+}
+
+// Value of a NoSlip, MovingWall, Inflow, Outflow or Pressure rule for
+// direction q, from x's own sums and own[qbar] (finv).
+template <class L, Mode M, int q>
+__device__ __forceinline__ typename Io<M, L>::Calc rule_value(const Cond& cd,
+                                                             const OwnSums<typename Io<M, L>::Calc>& os,
+                                                             typename Io<M, L>::Calc finv) {
+    using Calc = typename Io<M, L>::Calc;
+    constexpr int Q = L::Q;
+    switch (cd.kind) {
+        case BC_MOVINGWALL:  // boundary.hpp:55-65
+            if constexpr (M == Mode::EXACT64) {
+                double c_dot_u = 0;
+                c_dot_u += static_cast<double>(L::cx(q)) * cd.vel[0];
+                c_dot_u += static_cast<double>(L::cy(q)) * cd.vel[1];
+                c_dot_u += static_cast<double>(L::cz(q)) * cd.vel[2];
+                constexpr double cs = kCS;
+                return finv + 2.0 * weight<L>(q) * os.s * c_dot_u / (cs * cs);
+            } else {
+                const Calc rho = Calc(1) + os.s;
+                const Calc cu = static_cast<Calc>(L::cx(q) * cd.vel[0] + L::cy(q) * cd.vel[1] +
+                                                  L::cz(q) * cd.vel[2]);
+                return finv + static_cast<Calc>(6.0 * weight<L>(q)) * rho * cu;
+            }
+        case BC_INFLOW:  // boundary.hpp:176-178, feq precomputed on the host
+            return (M == Mode::EXACT64) ? static_cast<Calc>(cd.feq[q]) : static_cast<Calc>(cd.geq[q]);
+        case BC_OUTFLOW:     // boundary.hpp:140-147
+        case BC_PRESSURE:    // boundary.hpp:206-211
+            if constexpr (M == Mode::EXACT64) {
+                const double u[3] = {os.m0 / cd.rho, os.m1 / cd.rho, os.m2 / cd.rho};
+                const double fq = feq_exact<L, q>(cd.rho, u);
+                const double fb = feq_exact<L, Q - 1 - q>(cd.rho, u);
+                return fq + fb - finv;
+            } else {
+                const Calc rr = static_cast<Calc>(cd.rho);
+                const Calc ir = Calc(1) / rr;
+                const Calc ux = os.m0 * ir, uy = os.m1 * ir, uz = os.m2 * ir;
+                const Calc cu = Calc(L::cx(q)) * ux + Calc(L::cy(q)) * uy + Calc(L::cz(q)) * uz;
+                const Calc uu = ux * ux + uy * uy + uz * uz;
+                // (feq_q - w) + (feq_qbar - w): the odd part cancels
+                const Calc even = (rr - Calc(1)) + rr * (Calc(4.5) * cu * cu - Calc(1.5) * uu);
+                return static_cast<Calc>(2.0 * weight<L>(q)) * even - finv;
+            }
+        default:  // BC_NOSLIP, boundary.hpp:26-28
+            return finv;
     }
-    template <int q>
-    __device__ static Calc outflow(const Calc (&own)[Q], const Cond& cd) {
-        Calc jx = Calc(0), jy = Calc(0), jz = Calc(0);
-        static_for<0, Q>([&](auto kc) {
-            constexpr int k = decltype(kc)::value;
-            if constexpr (L::cx(k) > 0) jx += own[k];
-            if constexpr (L::cx(k) < 0) jx -= own[k];
-            if constexpr (L::cy(k) > 0) jy += own[k];
-            if constexpr (L::cy(k) < 0) jy -= own[k];
-            if constexpr (L::cz(k) > 0) jz += own[k];
-            if constexpr (L::cz(k) < 0) jz -= own[k];
-        });
-        const Calc rr = static_cast<Calc>(cd.rho);
-        const Calc ir = Calc(1) / rr;
-        const Calc ux = jx * ir, uy = jy * ir, uz = jz * ir;
-        const Calc cu = Calc(L::cx(q)) * ux + Calc(L::cy(q)) * uy + Calc(L::cz(q)) * uz;
-        const Calc uu = ux * ux + uy * uy + uz * uz;
-        // (feq_q - w) + (feq_qbar - w): the odd part cancels
-        const Calc even = (rr - Calc(1)) + rr * (Calc(4.5) * cu * cu - Calc(1.5) * uu);
-        return static_cast<Calc>(2.0 * weight<L>(q)) * even - own[Q - 1 - q];
-    }
-};
+}
 
 // Reflected direction for the FreeSlip chain (include/model.h velocity_index).
 template <class L>
@@ -302,56 +335,53 @@ __device__ __forceinline__ int dir_index(int cx, int cy, int cz) {
 
 // Value that fluid cell x = (x,y,z) pulls in direction q from the non-fluid
 // cell b = (sx,sy,sz) (element se) with condition cd: the value the reference's
-// rule writes into b[q] in collide pass 2 (include/boundary.hpp). Sets stale
-// when the rule writes nothing (FreeSlip with no fluid candidate); the value
-// returned is then b's stored value, which is what the reference reads too.
-template <class L, Mode M, int q>
-__device__ __forceinline__ typename Io<M, L>::Calc pull_nonfluid(
-    const StepArgs& a, const Cond& cd, const typename Io<M, L>::Calc (&own)[L::Q], int x, int y,
-    int z, int sx, int sy, int sz, int64_t se, bool& stale) {
+// rule writes into b[q] in collide pass 2 (include/boundary.hpp). os holds
+// the sums of x's own post-collision values and own(k) returns own value k.
+// Sets stale when the rule writes nothing (FreeSlip with no fluid candidate);
+// the value returned is then b's stored value, which is what the reference
+// reads too.
+template <class L, Mode M, int q, class OwnFn>
+__device__ __forceinline__ typename Io<M, L>::Calc pull_boundary(
+    const StepArgs& a, const Cond& cd, const OwnSums<typename Io<M, L>::Calc>& os, OwnFn own, int x,
+    int y, int z, int sx, int sy, int sz, int64_t se, bool& stale) {
     using IO = Io<M, L>;
     using T = typename IO::T;
-    using Calc = typename IO::Calc;
     constexpr int Q = L::Q;
     constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
     const FieldLayout& Ly = a.L;
     const T* __restrict__ src = static_cast<const T*>(a.src);
     stale = false;
-    switch (cd.kind) {
-        case BC_NOSLIP:  // boundary.hpp:26-28
-            return own[Q - 1 - q];
-        case BC_MOVINGWALL:
-            if constexpr (M == Mode::EXACT64)
-                return RulesExact<L>::template moving_wall<q>(own, cd);
-            else
-                return RulesShifted<L, Calc>::template moving_wall<q>(own, cd);
-        case BC_INFLOW:  // boundary.hpp:176-178, feq precomputed on the host
-            return (M == Mode::EXACT64) ? static_cast<Calc>(cd.feq[q]) : static_cast<Calc>(cd.geq[q]);
-        case BC_OUTFLOW:
-        case BC_PRESSURE:
-            if constexpr (M == Mode::EXACT64)
-                return RulesExact<L>::template outflow<q>(own, cd);
-            else
-                return RulesShifted<L, Calc>::template outflow<q>(own, cd);
-        case BC_FREESLIP: {  // boundary.hpp:91-110 with b = (sx,sy,sz), b + c_q = x
-            const int cand[6][3] = {{x, sy, sz}, {sx, y, sz}, {sx, sy, z},
-                                    {sx, y, z},  {x, sy, z},  {x, y, sz}};
-            const int refl[6][3] = {{-cx, cy, cz},  {cx, -cy, cz},  {cx, cy, -cz},
-                                    {cx, -cy, -cz}, {-cx, cy, -cz}, {-cx, -cy, cz}};
-            for (int j = 0; j < 6; ++j) {
-                const int64_t ce = cell_elem(Ly, cand[j][0], cand[j][1], cand[j][2]);
-                if (a.kind[ce] < KIND_COND0) {
-                    const int rq = dir_index<L>(refl[j][0], refl[j][1], refl[j][2]);
-                    return IO::to_calc_rt(src[rq * Ly.plane + ce], rq);
-                }
+    if (cd.kind == BC_FREESLIP) {  // boundary.hpp:91-110 with b = (sx,sy,sz), b + c_q = x
+        const int cand[6][3] = {{x, sy, sz}, {sx, y, sz}, {sx, sy, z},
+                                {sx, y, z},  {x, sy, z},  {x, y, sz}};
+        const int refl[6][3] = {{-cx, cy, cz},  {cx, -cy, cz},  {cx, cy, -cz},
+                                {cx, -cy, -cz}, {-cx, cy, -cz}, {-cx, -cy, cz}};
+        for (int j = 0; j < 6; ++j) {
+            const int64_t ce = cell_elem(Ly, cand[j][0], cand[j][1], cand[j][2]);
+            if (a.kind[ce] < KIND_COND0) {
+                const int rq = dir_index<L>(refl[j][0], refl[j][1], refl[j][2]);
+                return IO::to_calc_rt(src[rq * Ly.plane + ce], rq);
             }
-            stale = true;
-            return IO::template to_calc<q>(src[q * Ly.plane + se]);
         }
-        default:
-            stale = true;
-            return IO::template to_calc<q>(src[q * Ly.plane + se]);
+        stale = true;
+        return IO::template to_calc<q>(src[q * Ly.plane + se]);
     }
+    if (cd.kind < BC_NOSLIP || cd.kind > BC_PRESSURE) {
+        stale = true;
+        return IO::template to_calc<q>(src[q * Ly.plane + se]);
+    }
+    return rule_value<L, M, q>(cd, os, own(std::integral_constant<int, Q - 1 - q>{}));
+}
+
+// The same with x's own values already in registers.
+template <class L, Mode M, int q>
+__device__ __forceinline__ typename Io<M, L>::Calc pull_nonfluid(
+    const StepArgs& a, const Cond& cd, const typename Io<M, L>::Calc (&own)[L::Q], int x, int y,
+    int z, int sx, int sy, int sz, int64_t se, bool& stale) {
+    using Calc = typename Io<M, L>::Calc;
+    const OwnSums<Calc> os = own_sums<L, Calc>(own);
+    return pull_boundary<L, M, q>(a, cd, os, [&](auto kc) { return own[decltype(kc)::value]; }, x, y, z,
+                                  sx, sy, sz, se, stale);
 }
 
 // ---------------------------------------------------------------------------
@@ -367,11 +397,13 @@ __device__ __forceinline__ void near_cell(const StepArgs& a, int x, int y, int z
     const T* __restrict__ src = static_cast<const T*>(a.src);
     T* __restrict__ dst = static_cast<T*>(a.dst);
 
-    Calc own[Q];
-    static_for<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        own[q] = IO::template to_calc<q>(src[q * Ly.plane + e]);
-    });
+    // x's own post-collision values (the rules' inputs): sums now, single
+    // values reloaded when a rule needs one
+    const OwnSums<Calc> os = own_sums_at<L, M>(src, Ly.plane, e);
+    auto own = [&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        return IO::template to_calc<k>(src[k * Ly.plane + e]);
+    };
 
     Calc f[Q];
     static_for<0, Q>([&](auto qc) {
@@ -386,7 +418,7 @@ __device__ __forceinline__ void near_cell(const StepArgs& a, int x, int y, int z
             f[q] = IO::template to_calc<q>(src[q * Ly.plane + se]);
         } else {
             bool stale;
-            f[q] = pull_nonfluid<L, M, q>(a, a.conds[k - KIND_COND0], own, x, y, z, sx, sy, sz, se,
+            f[q] = pull_boundary<L, M, q>(a, a.conds[k - KIND_COND0], os, own, x, y, z, sx, sy, sz, se,
                                           stale);
         }
     });
@@ -410,6 +442,114 @@ __device__ __forceinline__ void near_cell(const StepArgs& a, int x, int y, int z
         constexpr int q = decltype(qc)::value;
         dst[q * Ly.plane + e] = IO::template from_calc<q>(f[q]);
     });
+}
+
+// ---------------------------------------------------------------------------
+// One interior cell of any kind, for kernels that hand every cell of a region
+// to a thread (the batched cell tiles). Non-fluid cells return at once. Every
+// direction is first pulled plainly (sources wrapped along periodic axes;
+// along a walled axis the source may be a ghost or solid cell); a NEAR cell
+// then replaces the values pulled from non-fluid sources by their boundary
+// values (as near_cell), and all lanes collide and store together. The bits
+// equal those of bulk_cells / near_cell for the same cell.
+template <class L, Mode M>
+__device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z) {
+    using IO = Io<M, L>;
+    using T = typename IO::T;
+    using Calc = typename IO::Calc;
+    constexpr int Q = L::Q;
+    const FieldLayout& Ly = a.L;
+    const int64_t e = cell_elem(Ly, x, y, z);
+    const uint8_t k = a.kind[e];
+    if (k >= KIND_COND0) return;
+    const T* __restrict__ src = static_cast<const T*>(a.src);
+    T* __restrict__ dst = static_cast<T*>(a.dst);
+
+    Calc f[Q];
+    // Away from the faces of periodic axes every source is a fixed element
+    // offset c_x + c_y px + c_z pxy away; on those faces the source wraps.
+    const bool wraps = (Ly.per[0] && (x == 1 || x == Ly.xl)) || (Ly.per[1] && (y == 1 || y == Ly.yl)) ||
+                       (Ly.per[2] && (z == 1 || z == Ly.zl));
+    if (!wraps) {
+        const T* __restrict__ at = src + e;
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
+            const int64_t d = cx + (cy > 0 ? Ly.px : (cy < 0 ? -Ly.px : 0)) +
+                              (cz > 0 ? Ly.pxy : (cz < 0 ? -Ly.pxy : 0));
+            f[q] = IO::template to_calc<q>(at[q * Ly.plane - d]);
+        });
+    } else {
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
+            const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
+            const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
+            const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
+            f[q] = IO::template to_calc<q>(src[q * Ly.plane + cell_elem(Ly, sx, sy, sz)]);
+        });
+    }
+    if (k == KIND_NEAR && !a.materialised) {
+        const OwnSums<Calc> os = own_sums_at<L, M>(src, Ly.plane, e);
+        auto own = [&](auto kc) {
+            constexpr int kk = decltype(kc)::value;
+            return IO::template to_calc<kk>(src[kk * Ly.plane + e]);
+        };
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
+            const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
+            const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
+            const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
+            const int64_t se = cell_elem(Ly, sx, sy, sz);
+            const uint8_t sk = a.kind[se];
+            if (sk >= KIND_COND0) {
+                bool stale;
+                f[q] = pull_boundary<L, M, q>(a, a.conds[sk - KIND_COND0], os, own, x, y, z, sx, sy, sz, se,
+                                              stale);
+            }
+        });
+    }
+
+    if constexpr (M == Mode::EXACT64) {
+        collide_exact<L>(f, a.tau);
+    } else if (a.alpha) {
+        const Calc alpha = static_cast<const Calc*>(a.alpha)[e];
+        collide_shifted<L, Calc, true, kRcp<M>>(f, static_cast<Calc>(a.omega), alpha,
+                                                static_cast<Calc>(a.tau * a.g[0]),
+                                                static_cast<Calc>(a.tau * a.g[1]),
+                                                static_cast<Calc>(a.tau * a.g[2]));
+    } else {
+        collide_shifted<L, Calc, false, kRcp<M>>(f, static_cast<Calc>(a.omega), Calc(0), Calc(0),
+                                                 Calc(0), Calc(0));
+    }
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        store_stream1<T>(dst + q * Ly.plane + e, IO::template from_calc<q>(f[q]));
+    });
+}
+
+// Cell-tile kernel: wavefront w of the grid takes the 64 cells
+// 64 (w mod tpp) .. + 63 (x fastest) of the z plane 1 + w / tpp.
+template <class L, Mode M>
+__global__ void __launch_bounds__(256) k_cells(StepArgs a, int tpp, double rxl) {
+    const int64_t w = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int z = 1 + static_cast<int>(w / tpp);
+    if (z > a.L.zl) return;
+    const int64_t c = (w % tpp) * 64 + (threadIdx.x & 63);
+    const int64_t xl = a.L.xl;
+    if (c >= xl * a.L.yl) return;
+    // row = c / xl: a double estimate is off by at most one, then corrected
+    int64_t y = static_cast<int64_t>(static_cast<double>(c) * rxl);
+    int64_t x = c - y * xl;
+    if (x < 0) {
+        --y;
+        x += xl;
+    } else if (x >= xl) {
+        ++y;
+        x -= xl;
+    }
+    any_cell<L, M>(a, 1 + static_cast<int>(x), 1 + static_cast<int>(y), z);
 }
 
 // Near-boundary kernel: one thread per listed NEAR cell.

@@ -20,7 +20,10 @@ __device__ __forceinline__ StepArgs view_args(const BatchView& v, int parity, in
 }
 
 // Persistent: each block walks tiles blockIdx.x, blockIdx.x + gridDim.x, ...
-template <class L, Mode M, int V>
+// With FUSED the tile also steps its NEAR cells (boundary rules) right after
+// its bulk cells, while their rows are still in cache; without it they are
+// left to k_batch_near.
+template <class L, Mode M, int V, bool FUSED>
 __global__ void __launch_bounds__(256) k_batch_bulk(BatchArgs b) {
     for (int64_t t = blockIdx.x; t < b.ntiles; t += gridDim.x) {
         const TileRef tr = b.tiles[t];
@@ -34,7 +37,49 @@ __global__ void __launch_bounds__(256) k_batch_bulk(BatchArgs b) {
         const int by = r % v.nby;
         const int z = 1 + r / v.nby;
         const StepArgs a = view_args(v, b.parity, b.materialised);
-        bulk_cells<L, M, V, true, false>(a, 1 + ((bx << bxl) + tx) * V, 1 + by * by_n + ty, z);
+        const int x0 = 1 + ((bx << bxl) + tx) * V, y = 1 + by * by_n + ty;
+        if constexpr (FUSED) {
+            bool nl[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) nl[i] = false;
+            bulk_cells<L, M, V, true, false>(a, x0, y, z, nl);
+            for (int i = 0; i < V; ++i)
+                if (nl[i]) near_cell<L, M>(a, x0 + i, y, z);
+        } else {
+            bulk_cells<L, M, V, true, false>(a, x0, y, z);
+        }
+    }
+}
+
+// Cell tiles: wavefront w walks tiles w, w + (waves in the grid), ...; each
+// lane takes one cell of its tile.
+// The tile index is wavefront-uniform; readfirstlane states it, so the tile
+// and the domain view are read into scalar registers once per wavefront.
+template <class L, Mode M>
+__global__ void __launch_bounds__(256) k_batch_cells(BatchArgs b) {
+    const int lane = threadIdx.x & 63;
+    const int nwaves = static_cast<int>(gridDim.x) * (static_cast<int>(blockDim.x) >> 6);
+    const int first = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * (blockDim.x >> 6) +
+                                                                      (threadIdx.x >> 6)));
+    for (int t = first; t < b.ntiles; t += nwaves) {
+        const TileRef tr = b.tiles[t];
+        const BatchView& v = b.views[tr.dom];
+        const int z = 1 + tr.local / v.tpp;
+        const int c = (tr.local % v.tpp) * 64 + lane;
+        if (c < v.L.xl * v.L.yl) {
+            // row = c / xl: float estimate (off by at most one below 2^22), then corrected
+            int y = static_cast<int>(static_cast<float>(c) * v.rxl);
+            int x = c - y * v.L.xl;
+            if (x < 0) {
+                --y;
+                x += v.L.xl;
+            } else if (x >= v.L.xl) {
+                ++y;
+                x -= v.L.xl;
+            }
+            const StepArgs a = view_args(v, b.parity, b.materialised);
+            any_cell<L, M>(a, 1 + x, 1 + y, z);
+        }
     }
 }
 
@@ -51,14 +96,24 @@ __global__ void __launch_bounds__(128) k_batch_near(BatchArgs b) {
 template <class L, Mode M, int... Vs>
 struct BatchImpl {
     template <int V>
-    static bool try_v(const BatchArgs& a, int V_, int grid, hipStream_t s) {
+    static bool try_v(const BatchArgs& a, int V_, bool fused, int grid, hipStream_t s) {
         if (V_ != V) return false;
-        hipLaunchKernelGGL((k_batch_bulk<L, M, V>), dim3(grid), dim3(256), 0, s, a);
+        if (fused)
+            hipLaunchKernelGGL((k_batch_bulk<L, M, V, true>), dim3(grid), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_batch_bulk<L, M, V, false>), dim3(grid), dim3(256), 0, s, a);
         return true;
     }
-    static hipError_t launch(const BatchArgs& a, int V, int grid_bulk, int grid_near, hipStream_t s) {
-        if (a.ntiles > 0 && !(try_v<Vs>(a, V, grid_bulk, s) || ...)) return hipErrorInvalidValue;
-        if (a.nnear > 0) hipLaunchKernelGGL((k_batch_near<L, M>), dim3(grid_near), dim3(128), 0, s, a);
+    static hipError_t launch(const BatchArgs& a, int V, int kernel, int grid_bulk, int grid_near,
+                             hipStream_t s) {
+        if (kernel == BK_CELLS) {
+            if (a.ntiles > 0) hipLaunchKernelGGL((k_batch_cells<L, M>), dim3(grid_bulk), dim3(256), 0, s, a);
+            return hipGetLastError();
+        }
+        const bool fused = kernel == BK_ROWS_FUSED;
+        if (a.ntiles > 0 && !(try_v<Vs>(a, V, fused, grid_bulk, s) || ...)) return hipErrorInvalidValue;
+        if (!fused && a.nnear > 0)
+            hipLaunchKernelGGL((k_batch_near<L, M>), dim3(grid_near), dim3(128), 0, s, a);
         return hipGetLastError();
     }
     static BatchOps make() {

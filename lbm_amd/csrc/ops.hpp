@@ -22,6 +22,12 @@ struct StepPlan {
     bool flags;     // domain has non-bulk interior cells: read the kind array
     bool brink;     // Brinkman field present
     bool has_bulk;  // at least one BULK cell
+    // Cell tiles (k_cells) instead of k_bulk + k_near: 64 consecutive cells of
+    // a z plane per wavefront, every cell kind in one pass.
+    bool cells;
+    int tpp;               // 64-cell tiles per z plane
+    int64_t nblocks_cells; // blocks of 4 wavefronts
+    double rxl;            // 1 / xl
 };
 
 struct ModeOps {

@@ -38,6 +38,16 @@ struct OpsImpl {
     static hipError_t step_impl(const StepArgs& a, const StepPlan& p, hipStream_t s,
                                 hipEvent_t* ev) {
         if (ev) (void)hipEventRecord(ev[0], s);
+        if (p.cells) {
+            hipLaunchKernelGGL((k_cells<L, M>), dim3(static_cast<uint32_t>(p.nblocks_cells)), dim3(256), 0, s, a,
+                               p.tpp, p.rxl);
+            if (ev) {
+                (void)hipEventRecord(ev[1], s);
+                (void)hipEventRecord(ev[2], s);
+                (void)hipEventRecord(ev[3], s);
+            }
+            return hipGetLastError();
+        }
         if (p.has_bulk) {
             bool done = (try_bulk<Vs>(a, p, s) || ...);
             if (!done) return hipErrorInvalidValue;

@@ -29,14 +29,27 @@ def bits(a):
     return np.ascontiguousarray(a).view(np.uint64)
 
 
+KERNELS = {"rows_separate": capi.BATCH_ROWS_SEPARATE, "rows_fused": capi.BATCH_ROWS_FUSED,
+           "cells": capi.BATCH_CELLS}
+
+
+@pytest.mark.parametrize("config", ["rows_separate", "rows_fused", "cells", "tuned"])
 @pytest.mark.parametrize("storage", ["f32", "f16", "f64s"])
-def test_batch_equals_single(storage):
+def test_batch_equals_single(storage, config):
+    """Every kernel form (row tiles with the near-boundary cells in a second
+    launch or in the same pass, 64-cell tiles per wavefront) and the
+    autotuner's pick: the same bits as each domain stepped alone."""
     specs = workloads.cavity_set(n=10, lmin=12, lmax=40, seed=7)
-    say(f"batch == single {storage}: {[s[:3] for s in specs]}")
+    say(f"batch == single {storage} {config}: {[s[:3] for s in specs]}")
     batch_doms = [cavity(*s[:4], storage) for s in specs]
     single_doms = [cavity(*s[:4], storage) for s in specs]
     b = capi.Batch(batch_doms)
-    say(f"  tune: {b.tune(2):.3f} ms/step, {b.info()}")
+    vec = 2 if storage != "f64s" else 1
+    if config == "tuned":
+        say(f"  tune: {b.tune(2):.3f} ms/step, {b.info()}")
+    else:
+        b.set_config(KERNELS[config], vec, 0, 8)
+        assert b.info()["kernel"] == KERNELS[config]
     b.step(37)
     b.sync()
     for d in single_doms:

@@ -315,12 +315,13 @@ VECS = {"f64": (1, 2), "f64s": (1, 2), "f32": (1, 2, 4), "f16": (2, 4, 8), "bf16
 @pytest.mark.parametrize("q", [15, 19, 27])
 def test_results_independent_of_cells_per_thread(storage, q):
     """Kernels round every product and sum as written (no FMA contraction), so
-    the number of cells a thread handles and the block width do not change a
-    bit of the result: a walled cavity (bulk and boundary kernels) and a
-    periodic Taylor-Green box with an odd x length (partial vectors)."""
+    the kernel form (cell tiles, or row kernels with V cells per thread plus
+    the near-boundary kernel) and the block width do not change a bit of the
+    result: a walled cavity and a periodic Taylor-Green box with an odd x
+    length (partial vectors)."""
     say(f"V independence {storage} q={q}")
     runs = {}
-    for v in VECS[storage]:
+    for v in (0,) + VECS[storage]:
         for bx in (0, 64):
             cav = capi.Domain(13, 11, 9, 0.6, q=q, storage=storage)
             scenario.cavity(13, 11, 9).apply(cav)
@@ -332,6 +333,8 @@ def test_results_independent_of_cells_per_thread(storage, q):
             for d in (cav, per):
                 d.set_tuning(v, bx)
                 d.step(23)
+            if v == 0:
+                assert cav.info()["cell_tiles"] == 1 and per.info()["cell_tiles"] == 0
             runs[(v, bx)] = (cav.get_pdf(), per.get_pdf())
     ref = runs[(VECS[storage][0], 0)]
     for key, (fc, fp) in runs.items():
