@@ -322,16 +322,20 @@ __device__ __forceinline__ typename Io<M, L>::Calc rule_value(const Cond& cd,
     }
 }
 
-// Reflected direction for the FreeSlip chain (include/model.h velocity_index).
+// FreeSlip candidates (include/boundary.hpp:93-110), in order: which of the
+// boundary cell's coordinates (x, y, z) are replaced by the fluid cell's, i.e.
+// which velocity components the candidate's population reflects.
+constexpr int kFsSwap[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+
+// Direction index of velocity (cx, cy, cz) (include/model.h velocity_index),
+// evaluated at compile time.
 template <class L>
-__device__ __forceinline__ int dir_index(int cx, int cy, int cz) {
-    int r = 0;
-    static_for<0, L::Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        if (L::cx(q) == cx && L::cy(q) == cy && L::cz(q) == cz) r = q;
-    });
-    return r;
+constexpr int dir_of(int cx, int cy, int cz) {
+    for (int q = 0; q < L::Q; ++q)
+        if (L::cx(q) == cx && L::cy(q) == cy && L::cz(q) == cz) return q;
+    return -1;
 }
+
 
 // Value that fluid cell x = (x,y,z) pulls in direction q from the non-fluid
 // cell b = (sx,sy,sz) (element se) with condition cd: the value the reference's
@@ -352,17 +356,26 @@ __device__ __forceinline__ typename Io<M, L>::Calc pull_boundary(
     const T* __restrict__ src = static_cast<const T*>(a.src);
     stale = false;
     if (cd.kind == BC_FREESLIP) {  // boundary.hpp:91-110 with b = (sx,sy,sz), b + c_q = x
-        const int cand[6][3] = {{x, sy, sz}, {sx, y, sz}, {sx, sy, z},
-                                {sx, y, z},  {x, sy, z},  {x, y, sz}};
-        const int refl[6][3] = {{-cx, cy, cz},  {cx, -cy, cz},  {cx, cy, -cz},
-                                {cx, -cy, -cz}, {-cx, cy, -cz}, {-cx, -cy, cz}};
-        for (int j = 0; j < 6; ++j) {
-            const int64_t ce = cell_elem(Ly, cand[j][0], cand[j][1], cand[j][2]);
-            if (a.kind[ce] < KIND_COND0) {
-                const int rq = dir_index<L>(refl[j][0], refl[j][1], refl[j][2]);
-                return IO::to_calc_rt(src[rq * Ly.plane + ce], rq);
+        // Candidates in the reference's order; candidate j replaces the
+        // coordinates flagged in kFsSwap[j] by x's and reflects those
+        // velocity components. The reflected directions are compile-time.
+        typename IO::Calc v{};
+        bool found = false;
+        static_for<0, 6>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            constexpr int rq = dir_of<L>(kFsSwap[j][0] ? -cx : cx, kFsSwap[j][1] ? -cy : cy,
+                                         kFsSwap[j][2] ? -cz : cz);
+            static_assert(rq >= 0, "reflected velocity outside the lattice");
+            if (!found) {
+                const int64_t ce = cell_elem(Ly, kFsSwap[j][0] ? x : sx, kFsSwap[j][1] ? y : sy,
+                                             kFsSwap[j][2] ? z : sz);
+                if (a.kind[ce] < KIND_COND0) {
+                    v = IO::template to_calc<rq>(src[rq * Ly.plane + ce]);
+                    found = true;
+                }
             }
-        }
+        });
+        if (found) return v;
         stale = true;
         return IO::template to_calc<q>(src[q * Ly.plane + se]);
     }

@@ -116,16 +116,6 @@ struct Io {
             return static_cast<T>(c);
         }
     }
-    // Runtime-direction variants for the cold paths.
-    __device__ __forceinline__ static Calc to_calc_rt(T v, int q) {
-        if constexpr (M == Mode::F16) {
-            return __half2float(v) / static_cast<float>(1u << scale_exp<L>(q));
-        } else if constexpr (M == Mode::BF16) {
-            return static_cast<float>(v) / static_cast<float>(1u << scale_exp<L>(q));
-        } else {
-            return static_cast<Calc>(v);
-        }
-    }
 };
 
 }  // namespace lbm_amd
