@@ -18,7 +18,9 @@ Two further legs ride on the same line:
   sweep, and their ratio;
 - "batch": 512 lid-driven D3Q19 fp32 cavities (edges log-uniform in
   [16, 96], Re uniform in [10, 1000] through per-domain tau) stepped 2000
-  times in one persistent, graph-captured launch per step after autotuning.
+  times in one persistent, graph-captured launch per step after autotuning,
+  then the batched adjoint of every cavity's kinetic energy with respect to
+  its lid velocity over 100 steps.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-adjoint] [--no-batch]
                     [--modes f32,f16,bf16]
@@ -163,13 +165,31 @@ def batch_leg(capi, steps: int = 2000, n: int = 512):
     b.step(20)
     b.sync()
     t = b.time_steps(steps)
+    # batched adjoint of each cavity's kinetic energy with respect to its lid
+    # velocity, on the same set (100 steps: sqrt(N) checkpoint slots of all
+    # 512 domains take ~100 GB)
+    adj_steps = 100
+    tape = capi.BatchTape(b, adj_steps)
+    J = tape.forward()
+    g = tape.backward()
+    tm = tape.timing()
+    # tau = 3 U L / Re + 1/2 reaches 0.5024 for small, high-Re cavities, where
+    # BGK is unstable: those diverge in the forward run and are counted here
+    stable = np.isfinite(J)
+    tape.close()
     b.close()
     for d in doms:
         d.close()
     return {"workload": f"{n} D3Q19 lid-driven cavities, edges log-uniform [16,96], Re U[10,1000], fp32",
             "steps": steps, "cells": t["cells"], "ms_per_step": round(t["ms"] / steps, 5),
-            "mlups": round(t["mlups"], 1), "vec": info["vec"], "blocks_per_cu": info["blocks_per_cu"],
-            "tiles": info["ntiles"], "near_cells": info["nnear"]}
+            "mlups": round(t["mlups"], 1), "kernel": info["kernel"], "vec": info["vec"],
+            "blocks_per_cu": info["blocks_per_cu"], "tiles": info["ntiles"], "near_cells": info["nnear"],
+            "adjoint": {"objective": "kinetic energy per cavity, gradient w.r.t. its lid velocity",
+                        "steps": adj_steps, "forward_ms": round(tm["forward_ms"], 3),
+                        "recompute_ms": round(tm["recompute_ms"], 3), "adjoint_ms": round(tm["adjoint_ms"], 3),
+                        "backward_over_forward": round((tm["recompute_ms"] + tm["adjoint_ms"]) / tm["forward_ms"], 3),
+                        "stable_cavities": int(stable.sum()),
+                        "gradients_finite_on_stable": bool(np.isfinite(g[stable]).all())}}
 
 
 def main() -> None:

@@ -259,6 +259,26 @@ enum lbm_batch_kernel {
  * tile or per four cell tiles). */
 int lbm_batch_set_config(lbm_batch* b, int kernel, int vec, int bx_policy, int blocks_per_cu);
 
+/* Batched adjoint (extension): every domain of a batch differentiated in
+ * lockstep, objective J_d = kinetic energy of domain d after nsteps steps
+ * from the domains' state at lbm_batch_adj_create, gradient of J_d with
+ * respect to domain d's moving-wall velocity (summed over its MovingWall
+ * conditions). Checkpoints every checkpoint_every steps (0: ceil(sqrt(N)));
+ * each domain holds N/k + k + 1 field-sized slots on the device. Backward
+ * recomputes each segment and steps the adjoint of all domains with one
+ * launch per step. D3Q19, F32 or F64S, NoSlip / MovingWall / Inflow. */
+typedef struct lbm_batch_tape lbm_batch_tape;
+int lbm_batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_every, lbm_batch_tape** out);
+int lbm_batch_adj_destroy(lbm_batch_tape* t);
+/* J receives one objective value per domain (or NULL); leaves the final state
+ * in the domains. */
+int lbm_batch_adj_forward(lbm_batch_tape* t, double* J);
+/* grad_wall receives 3 values per domain (x, y, z of dJ_d/du_wall). */
+int lbm_batch_adj_backward(lbm_batch_tape* t, double* grad_wall);
+/* out[0] ms in recompute launches, out[1] ms in adjoint launches, out[2]
+ * forward ms, out[3] steps. */
+int lbm_batch_adj_timing(lbm_batch_tape* t, double out[4]);
+
 /* Layout and size information. */
 typedef struct lbm_info {
     uint64_t ncells_interior;  /* xl*yl*zl */

@@ -44,6 +44,7 @@ UNITS = {
     "mode_adjoint.hip": NOFMA,
     "capi_batch.hip": [],
     "mode_batch.hip": NOFMA,
+    "mode_batch_adjoint.hip": NOFMA,
     "mode_exact.hip": NOFMA,
     "mode_fast_q15.hip": NOFMA,
     "mode_fast_q19.hip": NOFMA,

@@ -84,6 +84,11 @@ SIGNATURES = [
     ("lbm_batch_time_steps", c_int, [c_void_p, c_uint64, POINTER(c_double)]),
     ("lbm_batch_get_info", c_int, [c_void_p, c_void_p]),
     ("lbm_batch_set_config", c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    ("lbm_batch_adj_create", c_int, [c_void_p, c_uint64, c_uint64, POINTER(c_void_p)]),
+    ("lbm_batch_adj_destroy", c_int, [c_void_p]),
+    ("lbm_batch_adj_forward", c_int, [c_void_p, POINTER(c_double)]),
+    ("lbm_batch_adj_backward", c_int, [c_void_p, POINTER(c_double)]),
+    ("lbm_batch_adj_timing", c_int, [c_void_p, POINTER(c_double)]),
     ("lbm_get_info", c_int, [c_void_p, POINTER(Info)]),
     ("lbm_set_tuning", c_int, [c_void_p, c_int, c_int]),
 ]
@@ -380,3 +385,41 @@ class Batch:
         i = BatchInfo()
         _check(load().lbm_batch_get_info(self._h, ctypes.cast(byref(i), c_void_p)))
         return {k: getattr(i, k) for k, _ in BatchInfo._fields_}
+
+
+class BatchTape:
+    """Batched adjoint over a Batch (lbm_batch_adj_*): forward() -> J per
+    domain, backward() -> (ndomains, 3) lid-velocity gradients."""
+
+    def __init__(self, batch: Batch, nsteps: int, checkpoint_every: int = 0):
+        self.batch = batch
+        h = c_void_p()
+        _check(load().lbm_batch_adj_create(batch._h, nsteps, checkpoint_every, byref(h)))
+        self._h = h
+        self.n = len(batch.domains)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().lbm_batch_adj_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self) -> np.ndarray:
+        j = np.zeros(self.n)
+        _check(load().lbm_batch_adj_forward(self._h, _dptr(j)))
+        return j
+
+    def backward(self) -> np.ndarray:
+        g = np.zeros((self.n, 3))
+        _check(load().lbm_batch_adj_backward(self._h, _dptr(g)))
+        return g
+
+    def timing(self) -> dict:
+        out = np.zeros(4)
+        _check(load().lbm_batch_adj_timing(self._h, _dptr(out)))
+        return {"recompute_ms": out[0], "adjoint_ms": out[1], "forward_ms": out[2], "steps": int(out[3])}

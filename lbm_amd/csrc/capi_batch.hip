@@ -10,12 +10,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
 #include "batch.hpp"
+#include "batch_adjoint.hpp"
 #include "domain_impl.hpp"
 #include "lbm_hip.h"
+#include "ops_adjoint.hpp"
 
 using namespace lbm_amd;
 
@@ -413,6 +416,264 @@ int lbm_batch_get_info(lbm_batch* b, lbm_batch_info* info) {
     info->bx_policy = b->bx_policy;
     info->ncu = b->ncu;
     info->kernel = b->kernel;
+    return LBM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched adjoint: all domains of a batch differentiated together, kinetic
+// energy objective per domain, gradients with respect to each domain's
+// moving-wall velocity. Forward: the batch's graph-captured steps with a
+// checkpoint of every domain every k steps. Backward: per segment, one
+// recompute launch and one adjoint launch per step for all domains.
+
+struct lbm_batch_tape {
+    lbm_batch* b = nullptr;
+    const BatchAdjOps* ops = nullptr;
+    const AdjOps* aops = nullptr;
+    uint64_t T = 0, k = 0, nck = 0;
+    int nslots = 0;
+    std::vector<void*> chunk;
+    double* d_gw = nullptr;
+    std::vector<BatchAdjView> views;
+    BatchAdjView* d_views = nullptr;
+    TileRef* d_tiles = nullptr;
+    int64_t ntiles = 0;
+    double* d_out = nullptr;
+    bool start_mat = false;
+    bool forward_done = false;
+    double timing[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+void btape_free(lbm_batch_tape* t) {
+    for (void* p : t->chunk) (void)hipFree(p);
+    t->chunk.clear();
+    (void)hipFree(t->d_gw);
+    (void)hipFree(t->d_views);
+    (void)hipFree(t->d_tiles);
+    (void)hipFree(t->d_out);
+    t->d_gw = nullptr;
+    t->d_views = nullptr;
+    t->d_tiles = nullptr;
+    t->d_out = nullptr;
+}
+
+// slots: checkpoints 0 .. nck-1, segment states j = 1 .. k-1, two mu buffers
+int seg_slot(const lbm_batch_tape* t, uint64_t j) { return static_cast<int>(t->nck + j - 1); }
+int mu_slot(const lbm_batch_tape* t, int i) { return static_cast<int>(t->nck + t->k - 1) + i; }
+
+int grid_for(const lbm_batch_tape* t) {
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((t->ntiles + 3) / 4, 0x7fffffff)));
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_every, lbm_batch_tape** out) {
+    if (!b || !out) return fail(LBM_ERR_ARG, "null argument");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(b->device));
+    lbm_domain* d0 = b->doms[0];
+    const BatchAdjOps* ops = batch_adj_ops(d0->Q, d0->mode);
+    const AdjOps* aops = adj_ops(d0->Q, d0->mode);
+    if (!ops || !aops) return fail(LBM_ERR_UNSUPPORTED, "batched adjoint needs D3Q19 with F32 or F64S storage");
+    for (const lbm_domain* d : b->doms)
+        for (const HostCond& c : d->conds)
+            if (c.kind != LBM_BC_NOSLIP && c.kind != LBM_BC_MOVINGWALL && c.kind != LBM_BC_INFLOW)
+                return fail(LBM_ERR_UNSUPPORTED, "adjoint supports NoSlip, MovingWall and Inflow boundaries");
+    if (nsteps < 1 || nsteps > 10000000) return fail(LBM_ERR_ARG, "nsteps in [1, 1e7]");
+    int parity, mat;
+    if (int rc = check_parity(b, parity, mat)) return rc;
+    if (int rc = sync_domains(b)) return rc;
+
+    lbm_batch_tape* t = new lbm_batch_tape();
+    t->b = b;
+    t->ops = ops;
+    t->aops = aops;
+    t->T = nsteps;
+    t->k = checkpoint_every ? checkpoint_every
+                            : static_cast<uint64_t>(std::ceil(std::sqrt(static_cast<double>(nsteps))));
+    if (t->k > t->T) t->k = t->T;
+    t->nck = (t->T + t->k - 1) / t->k;
+    t->nslots = static_cast<int>(t->nck + t->k + 1);
+    t->start_mat = mat != 0;
+    auto bail = [&](int code, const std::string& m) {
+        btape_free(t);
+        delete t;
+        return fail(code, m);
+    };
+    if (t->nck + t->k + 1 > 0x7fffffff) return bail(LBM_ERR_ARG, "too many checkpoint slots");
+
+    std::vector<TileRef> tiles;
+    int64_t gw_elems = 0;
+    std::vector<int64_t> gw_off;
+    for (size_t di = 0; di < b->doms.size(); ++di) {
+        lbm_domain* d = b->doms[di];
+        void* c = nullptr;
+        if (hipMalloc(&c, static_cast<size_t>(t->nslots) * d->field_bytes) != hipSuccess)
+            return bail(LBM_ERR_ALLOC, "checkpoint slots do not fit in device memory (lower nsteps or raise checkpoint_every)");
+        t->chunk.push_back(c);
+        BatchAdjView v{};
+        v.L = d->L;
+        v.chunk = c;
+        v.slot_elems = static_cast<int64_t>(d->field_bytes / d->esize);
+        v.gfinal = d->buf[d->cur];
+        v.kind = d->d_kind;
+        v.conds = d->d_conds;
+        v.tau = d->desc.tau;
+        v.omega = 1.0 / d->desc.tau;
+        v.tpp = static_cast<int>((static_cast<int64_t>(d->L.xl) * d->L.yl + 63) / 64);
+        v.rxl = 1.0 / static_cast<double>(d->L.xl);
+        t->views.push_back(v);
+        gw_off.push_back(gw_elems);
+        gw_elems += 3 * d->L.plane;
+        const int64_t nt = static_cast<int64_t>(v.tpp) * d->L.zl;
+        if (static_cast<int64_t>(tiles.size()) + nt > 0x3fffffff) return bail(LBM_ERR_UNSUPPORTED, "too many tiles");
+        for (int64_t i = 0; i < nt; ++i) tiles.push_back(TileRef{static_cast<int>(di), static_cast<int>(i)});
+        // checkpoint 0: the domain's current state
+        if (hipMemcpyAsync(c, d->buf[d->cur], d->field_bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
+            return bail(LBM_ERR_HIP, "checkpoint copy failed");
+    }
+    if (hipMalloc(&t->d_gw, sizeof(double) * gw_elems) != hipSuccess) return bail(LBM_ERR_ALLOC, "gradient buffers");
+    for (size_t di = 0; di < t->views.size(); ++di) t->views[di].gw = t->d_gw + gw_off[di];
+    t->ntiles = static_cast<int64_t>(tiles.size());
+    if (hipMalloc(&t->d_views, sizeof(BatchAdjView) * t->views.size()) != hipSuccess ||
+        hipMalloc(&t->d_tiles, sizeof(TileRef) * tiles.size()) != hipSuccess ||
+        hipMalloc(&t->d_out, sizeof(double) * 3 * t->views.size()) != hipSuccess)
+        return bail(LBM_ERR_ALLOC, "tape tables");
+    if (hipMemcpy(t->d_tiles, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return bail(LBM_ERR_HIP, "tape setup");
+    *out = t;
+    return LBM_OK;
+}
+
+int lbm_batch_adj_destroy(lbm_batch_tape* t) {
+    if (!t) return LBM_OK;
+    (void)hipSetDevice(t->b->device);
+    (void)hipStreamSynchronize(t->b->stream);
+    btape_free(t);
+    delete t;
+    return LBM_OK;
+}
+
+int lbm_batch_adj_forward(lbm_batch_tape* t, double* J) {
+    if (!t) return fail(LBM_ERR_ARG, "null tape");
+    lbm_batch* b = t->b;
+    HIPCHK(hipSetDevice(b->device));
+    if (int rc = sync_domains(b)) return rc;
+    for (size_t di = 0; di < b->doms.size(); ++di) {
+        lbm_domain* d = b->doms[di];
+        HIPCHK(hipMemcpyAsync(d->buf[0], t->chunk[di], d->field_bytes, hipMemcpyDeviceToDevice, b->stream));
+        d->cur = 0;
+        d->materialised = t->start_mat;
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, b->stream));
+    for (uint64_t si = 0; si < t->nck; ++si) {
+        const uint64_t steps = std::min(t->k, t->T - si * t->k);
+        int parity, mat;
+        if (int rc = check_parity(b, parity, mat)) return rc;
+        if (int rc = ensure_graph(b, steps, parity, mat)) return rc;
+        HIPCHK(hipGraphLaunch(b->exec, b->stream));
+        advance_domains(b, steps);
+        if (si + 1 < t->nck)
+            for (size_t di = 0; di < b->doms.size(); ++di) {
+                lbm_domain* d = b->doms[di];
+                char* dst = static_cast<char*>(t->chunk[di]) + (si + 1) * d->field_bytes;
+                HIPCHK(hipMemcpyAsync(dst, d->buf[d->cur], d->field_bytes, hipMemcpyDeviceToDevice, b->stream));
+            }
+    }
+    HIPCHK(hipEventRecord(e1, b->stream));
+    for (size_t di = 0; di < b->doms.size(); ++di) t->views[di].gfinal = b->doms[di]->buf[b->doms[di]->cur];
+    HIPCHK(hipMemcpyAsync(t->d_views, t->views.data(), sizeof(BatchAdjView) * t->views.size(), hipMemcpyHostToDevice,
+                          b->stream));
+    for (size_t di = 0; di < b->doms.size(); ++di) {
+        lbm_domain* d = b->doms[di];
+        HIPCHK(t->aops->objective(d->L, d->buf[d->cur], d->d_kind, OBJ_KINETIC, 0, nullptr, d->d_partial, d->nred,
+                                  d->d_out4, b->stream));
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    t->timing[2] = ms;
+    t->timing[3] = static_cast<double>(t->T);
+    if (J)
+        for (size_t di = 0; di < b->doms.size(); ++di) {
+            double o[4];
+            HIPCHK(hipMemcpy(o, b->doms[di]->d_out4, sizeof(o), hipMemcpyDeviceToHost));
+            J[di] = o[0];
+        }
+    t->forward_done = true;
+    return LBM_OK;
+}
+
+int lbm_batch_adj_backward(lbm_batch_tape* t, double* grad_wall) {
+    if (!t) return fail(LBM_ERR_ARG, "null tape");
+    if (!t->forward_done) return fail(LBM_ERR_STATE, "lbm_batch_adj_backward needs a preceding lbm_batch_adj_forward");
+    lbm_batch* b = t->b;
+    HIPCHK(hipSetDevice(b->device));
+    int64_t gw_elems = 0;
+    for (const lbm_domain* d : b->doms) gw_elems += 3 * d->L.plane;
+    HIPCHK(hipMemsetAsync(t->d_gw, 0, sizeof(double) * gw_elems, b->stream));
+    BatchAdjArgs a{};
+    a.views = t->d_views;
+    a.tiles = t->d_tiles;
+    a.ntiles = t->ntiles;
+    const int grid = grid_for(t);
+    std::vector<hipEvent_t> ev(3 * t->nck, nullptr);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    for (uint64_t si = t->nck; si-- > 0;) {
+        const uint64_t start = si * t->k;
+        const uint64_t end = std::min(start + t->k, t->T);
+        auto state = [&](uint64_t s) { return s == start ? static_cast<int>(si) : seg_slot(t, s - start); };
+        HIPCHK(hipEventRecord(ev[3 * si + 0], b->stream));
+        for (uint64_t s = start + 1; s < end; ++s) {
+            a.src_slot = state(s - 1);
+            a.dst_slot = state(s);
+            a.materialised = (s == 1 && t->start_mat) ? 1 : 0;
+            HIPCHK(t->ops->recompute(a, grid, b->stream));
+        }
+        HIPCHK(hipEventRecord(ev[3 * si + 1], b->stream));
+        for (uint64_t s = end; s > start; --s) {
+            a.gprev_slot = state(s - 1);
+            a.mu_in_slot = mu_slot(t, static_cast<int>((s + 1) & 1));
+            a.mu_out_slot = mu_slot(t, static_cast<int>(s & 1));
+            a.terminal = (s == t->T) ? 1 : 0;
+            a.materialised = (s == 1 && t->start_mat) ? 1 : 0;
+            HIPCHK(t->ops->backward(a, grid, b->stream));
+        }
+        HIPCHK(hipEventRecord(ev[3 * si + 2], b->stream));
+    }
+    HIPCHK(t->ops->reduce(t->d_views, static_cast<int>(t->views.size()), t->d_out, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    double rec = 0, adj = 0;
+    for (uint64_t si = 0; si < t->nck; ++si) {
+        float ms;
+        HIPCHK(hipEventElapsedTime(&ms, ev[3 * si], ev[3 * si + 1]));
+        rec += ms;
+        HIPCHK(hipEventElapsedTime(&ms, ev[3 * si + 1], ev[3 * si + 2]));
+        adj += ms;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    t->timing[0] = rec;
+    t->timing[1] = adj;
+    if (grad_wall)
+        HIPCHK(hipMemcpy(grad_wall, t->d_out, sizeof(double) * 3 * t->views.size(), hipMemcpyDeviceToHost));
+    return LBM_OK;
+}
+
+int lbm_batch_adj_timing(lbm_batch_tape* t, double out[4]) {
+    if (!t || !out) return fail(LBM_ERR_ARG, "null argument");
+    for (int i = 0; i < 4; ++i) out[i] = t->timing[i];
     return LBM_OK;
 }
 

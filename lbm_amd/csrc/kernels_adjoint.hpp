@@ -122,25 +122,14 @@ __device__ __forceinline__ Calc adjoint_collide(const Calc (&h)[L::Q], const Cal
         return Calc(0);
 }
 
-// Backward step for BULK cells (all Q neighbours fluid): one cell per thread,
-// blocks and grid as k_bulk with V = 1.
+// Backward step of one BULK cell (all Q neighbours fluid).
 template <class L, Mode M, bool BRINK>
-__global__ void __launch_bounds__(256) k_adj_bulk(AdjArgs a, int nbx, int nby) {
+__device__ __forceinline__ void adj_bulk_cell(const AdjArgs& a, int x, int y, int z, int64_t e) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
     constexpr int Q = L::Q;
     const FieldLayout& Ly = a.L;
-    const int b = blockIdx.x;
-    const int bx = b % nbx;
-    const int t = b / nbx;
-    const int by = t % nby;
-    const int z = 1 + t / nby;
-    const int x = 1 + bx * blockDim.x + threadIdx.x;
-    const int y = 1 + by * blockDim.y + threadIdx.y;
-    if (x > Ly.xl || y > Ly.yl) return;
-    const int64_t e = cell_elem(Ly, x, y, z);
-    if (a.kind[e] != KIND_BULK) return;
     const int xm = wrap1(x - 1, Ly.xl), xp = wrap1(x + 1, Ly.xl);
     const int ym = wrap1(y - 1, Ly.yl), yp = wrap1(y + 1, Ly.yl);
     const int zm = wrap1(z - 1, Ly.zl), zp = wrap1(z + 1, Ly.zl);
@@ -190,19 +179,35 @@ __global__ void __launch_bounds__(256) k_adj_bulk(AdjArgs a, int nbx, int nby) {
     if constexpr (BRINK) a.grad_alpha[e] += static_cast<double>(da);
 }
 
-// Backward step for NEAR cells (listed), with the adjoint boundary rules of
-// NoSlip, MovingWall and Inflow sources (other kinds are rejected on the host).
+// Backward step for BULK cells: one cell per thread, blocks and grid as
+// k_bulk with V = 1.
 template <class L, Mode M, bool BRINK>
-__global__ void __launch_bounds__(128) k_adj_near(AdjArgs a) {
+__global__ void __launch_bounds__(256) k_adj_bulk(AdjArgs a, int nbx, int nby) {
+    const FieldLayout& Ly = a.L;
+    const int b = blockIdx.x;
+    const int bx = b % nbx;
+    const int t = b / nbx;
+    const int by = t % nby;
+    const int z = 1 + t / nby;
+    const int x = 1 + bx * blockDim.x + threadIdx.x;
+    const int y = 1 + by * blockDim.y + threadIdx.y;
+    if (x > Ly.xl || y > Ly.yl) return;
+    const int64_t e = cell_elem(Ly, x, y, z);
+    if (a.kind[e] != KIND_BULK) return;
+    adj_bulk_cell<L, M, BRINK>(a, x, y, z, e);
+}
+
+// Backward step of one fluid cell of any kind, with the adjoint boundary
+// rules of NoSlip, MovingWall and Inflow sources (other kinds are rejected on
+// the host). gw (3 doubles) accumulates dJ/du_wall of this cell's moving-wall
+// links.
+template <class L, Mode M, bool BRINK>
+__device__ __forceinline__ void adj_cell(const AdjArgs& a, int x, int y, int z, double* __restrict__ gw) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
     constexpr int Q = L::Q;
     const FieldLayout& Ly = a.L;
-    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (i >= a.nb) return;
-    const CellCoord c = a.blist[i];
-    const int x = c.x, y = c.y, z = c.z;
     const int64_t e = cell_elem(Ly, x, y, z);
     const T* __restrict__ gp = static_cast<const T*>(a.gprev);
     const Calc* __restrict__ mi = static_cast<const Calc*>(a.mu_in);
@@ -293,6 +298,7 @@ __global__ void __launch_bounds__(128) k_adj_near(AdjArgs a) {
 
     // dJ/du_wall of this step: h_q = g_qbar + 6 w_q rho c_q.u_w on moving-wall links
     if (!a.materialised) {
+        bool walls = false;
         double gw0 = 0, gw1 = 0, gw2 = 0;
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
@@ -305,12 +311,157 @@ __global__ void __launch_bounds__(128) k_adj_near(AdjArgs a) {
                 gw0 += f * L::cx(q);
                 gw1 += f * L::cy(q);
                 gw2 += f * L::cz(q);
+                walls = true;
             }
         });
-        a.grad_wall[3 * i + 0] += gw0;
-        a.grad_wall[3 * i + 1] += gw1;
-        a.grad_wall[3 * i + 2] += gw2;
+        if (walls) {
+            gw[0] += gw0;
+            gw[1] += gw1;
+            gw[2] += gw2;
+        }
     }
+}
+
+// Backward step of one fluid cell of any kind for kernels that give every
+// cell of a region to a lane (batched cell tiles). All lanes pull h and
+// reverse-pull lambda plainly, collide and store together; a NEAR lane then
+// replaces the values on its boundary links and adds the wall terms, with the
+// operations of adj_cell in the same order, so mu and the gradients carry
+// the same bits as k_adj_bulk / k_adj_near.
+template <class L, Mode M, bool BRINK>
+__device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int z, int64_t e, bool near,
+                                             double* __restrict__ gw) {
+    using IO = Io<M, L>;
+    using T = typename IO::T;
+    using Calc = typename IO::Calc;
+    constexpr int Q = L::Q;
+    const FieldLayout& Ly = a.L;
+    const T* __restrict__ gp = static_cast<const T*>(a.gprev);
+    const Calc* __restrict__ mi = static_cast<const Calc*>(a.mu_in);
+    const bool wraps = (Ly.per[0] && (x == 1 || x == Ly.xl)) || (Ly.per[1] && (y == 1 || y == Ly.yl)) ||
+                       (Ly.per[2] && (z == 1 || z == Ly.zl));
+    // element offset of x + s c_q (s = -1: source, +1: consumer)
+    auto nb_elem = [&](auto qc, int sgn) -> int64_t {
+        constexpr int q = decltype(qc)::value;
+        if (!wraps)
+            return e + sgn * (L::cx(q) + L::cy(q) * Ly.px + L::cz(q) * Ly.pxy);
+        return cell_elem(Ly, wrap_if(x + sgn * L::cx(q), Ly.xl, Ly.per[0]),
+                         wrap_if(y + sgn * L::cy(q), Ly.yl, Ly.per[1]), wrap_if(z + sgn * L::cz(q), Ly.zl, Ly.per[2]));
+    };
+
+    Calc h[Q];
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        h[q] = IO::template to_calc<q>(gp[q * Ly.plane + nb_elem(qc, -1)]);
+    });
+    if (near && !a.materialised) {
+        StepArgs sa{};
+        sa.L = Ly;
+        sa.src = a.gprev;
+        sa.kind = a.kind;
+        sa.conds = a.conds;
+        const OwnSums<Calc> os = own_sums_at<L, M>(gp, Ly.plane, e);
+        auto own = [&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            return IO::template to_calc<k>(gp[k * Ly.plane + e]);
+        };
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int sx = wrap_if(x - L::cx(q), Ly.xl, Ly.per[0]);
+            const int sy = wrap_if(y - L::cy(q), Ly.yl, Ly.per[1]);
+            const int sz = wrap_if(z - L::cz(q), Ly.zl, Ly.per[2]);
+            const int64_t se = cell_elem(Ly, sx, sy, sz);
+            const uint8_t k = a.kind[se];
+            if (k >= KIND_COND0) {
+                bool stale;
+                h[q] = pull_boundary<L, M, q>(sa, a.conds[k - KIND_COND0], os, own, x, y, z, sx, sy, sz, se, stale);
+            }
+        });
+    }
+
+    Calc lam[Q];
+    if (a.terminal) {
+        const T* __restrict__ gc = static_cast<const T*>(a.gcur);
+        Calc gf[Q];
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            gf[q] = IO::template to_calc<q>(gc[q * Ly.plane + e]);
+        });
+        objective_grad<L, Calc>(a, x, y, z, gf, lam);
+    } else {
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            lam[q] = mi[q * Ly.plane + nb_elem(qc, 1)];
+        });
+        if (near) {
+            Calc wall_term = Calc(0);
+            static_for<0, Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const uint8_t k = a.kind[nb_elem(qc, 1)];
+                if (k >= KIND_COND0) {
+                    const int kind = a.conds[k - KIND_COND0].kind;
+                    lam[q] = (kind == BC_NOSLIP || kind == BC_MOVINGWALL) ? mi[(Q - 1 - q) * Ly.plane + e] : Calc(0);
+                }
+                const uint8_t ks = a.kind[nb_elem(qc, -1)];
+                if (ks >= KIND_COND0 && ks < KIND_INERT) {
+                    const Cond& cd = a.conds[ks - KIND_COND0];
+                    if (cd.kind == BC_MOVINGWALL) {
+                        const double cu = L::cx(q) * cd.vel[0] + L::cy(q) * cd.vel[1] + L::cz(q) * cd.vel[2];
+                        wall_term += mi[q * Ly.plane + e] * static_cast<Calc>(6.0 * weight<L>(q) * cu);
+                    }
+                }
+            });
+            static_for<0, Q>([&](auto qc) { lam[decltype(qc)::value] += wall_term; });
+        }
+    }
+
+    Calc alpha = Calc(0);
+    if constexpr (BRINK) alpha = static_cast<const Calc*>(a.alpha)[e];
+    Calc mu[Q];
+    const Calc da = adjoint_collide<L, Calc, BRINK>(
+        h, lam, mu, static_cast<Calc>(a.omega), alpha, static_cast<Calc>(a.tau * a.g[0]),
+        static_cast<Calc>(a.tau * a.g[1]), static_cast<Calc>(a.tau * a.g[2]));
+    Calc* __restrict__ mo = static_cast<Calc*>(a.mu_out);
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        mo[q * Ly.plane + e] = mu[q];
+    });
+    if constexpr (BRINK) a.grad_alpha[e] += static_cast<double>(da);
+
+    if (near && !a.materialised) {
+        Calc rho_own = Calc(1);
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            rho_own += IO::template to_calc<q>(gp[q * Ly.plane + e]);
+        });
+        bool walls = false;
+        double gw0 = 0, gw1 = 0, gw2 = 0;
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const uint8_t ks = a.kind[nb_elem(qc, -1)];
+            if (ks >= KIND_COND0 && ks < KIND_INERT && a.conds[ks - KIND_COND0].kind == BC_MOVINGWALL) {
+                const double f = static_cast<double>(mu[q] * rho_own) * 6.0 * weight<L>(q);
+                gw0 += f * L::cx(q);
+                gw1 += f * L::cy(q);
+                gw2 += f * L::cz(q);
+                walls = true;
+            }
+        });
+        if (walls) {
+            gw[0] += gw0;
+            gw[1] += gw1;
+            gw[2] += gw2;
+        }
+    }
+}
+
+// Backward step for NEAR cells (listed): gradients per NEAR cell index.
+template <class L, Mode M, bool BRINK>
+__global__ void __launch_bounds__(128) k_adj_near(AdjArgs a) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= a.nb) return;
+    const CellCoord c = a.blist[i];
+    adj_cell<L, M, BRINK>(a, c.x, c.y, c.z, a.grad_wall + 3 * i);
 }
 
 // Objective value over interior fluid cells in fp64 (component 0), and for
