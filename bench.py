@@ -149,7 +149,7 @@ def adjoint_leg(capi, steps: int = 2000):
             "backward_over_forward": round((tm["recompute_ms"] + tm["adjoint_ms"]) / fwd, 3)}
 
 
-def batch_leg(capi, steps: int = 2000, n: int = 512):
+def batch_leg(capi, steps: int = 2000, n: int = 512, adj_steps: int = 100):
     """512 uneven lid-driven cavities in one persistent graph-captured launch."""
     from lbm_amd import scenario, workloads
     specs = workloads.cavity_set(n=n)
@@ -165,10 +165,19 @@ def batch_leg(capi, steps: int = 2000, n: int = 512):
     b.step(20)
     b.sync()
     t = b.time_steps(steps)
+    out = {"workload": f"{n} D3Q19 lid-driven cavities, edges log-uniform [16,96], Re U[10,1000], fp32",
+           "steps": steps, "cells": t["cells"], "ms_per_step": round(t["ms"] / steps, 5),
+           "mlups": round(t["mlups"], 1), "kernel": info["kernel"], "vec": info["vec"],
+           "blocks_per_cu": info["blocks_per_cu"], "tiles": info["ntiles"], "near_cells": info["nnear"],
+           "adjoint": None}
+    if adj_steps <= 0:
+        b.close()
+        for d in doms:
+            d.close()
+        return out
     # batched adjoint of each cavity's kinetic energy with respect to its lid
     # velocity, on the same set (100 steps: sqrt(N) checkpoint slots of all
     # 512 domains take ~100 GB)
-    adj_steps = 100
     tape = capi.BatchTape(b, adj_steps)
     J = tape.forward()
     g = tape.backward()
@@ -180,16 +189,13 @@ def batch_leg(capi, steps: int = 2000, n: int = 512):
     b.close()
     for d in doms:
         d.close()
-    return {"workload": f"{n} D3Q19 lid-driven cavities, edges log-uniform [16,96], Re U[10,1000], fp32",
-            "steps": steps, "cells": t["cells"], "ms_per_step": round(t["ms"] / steps, 5),
-            "mlups": round(t["mlups"], 1), "kernel": info["kernel"], "vec": info["vec"],
-            "blocks_per_cu": info["blocks_per_cu"], "tiles": info["ntiles"], "near_cells": info["nnear"],
-            "adjoint": {"objective": "kinetic energy per cavity, gradient w.r.t. its lid velocity",
-                        "steps": adj_steps, "forward_ms": round(tm["forward_ms"], 3),
-                        "recompute_ms": round(tm["recompute_ms"], 3), "adjoint_ms": round(tm["adjoint_ms"], 3),
-                        "backward_over_forward": round((tm["recompute_ms"] + tm["adjoint_ms"]) / tm["forward_ms"], 3),
-                        "stable_cavities": int(stable.sum()),
-                        "gradients_finite_on_stable": bool(np.isfinite(g[stable]).all())}}
+    out["adjoint"] = {"objective": "kinetic energy per cavity, gradient w.r.t. its lid velocity",
+                      "steps": adj_steps, "forward_ms": round(tm["forward_ms"], 3),
+                      "recompute_ms": round(tm["recompute_ms"], 3), "adjoint_ms": round(tm["adjoint_ms"], 3),
+                      "backward_over_forward": round((tm["recompute_ms"] + tm["adjoint_ms"]) / tm["forward_ms"], 3),
+                      "stable_cavities": int(stable.sum()),
+                      "gradients_finite_on_stable": bool(np.isfinite(g[stable]).all())}
+    return out
 
 
 def main() -> None:
@@ -202,6 +208,8 @@ def main() -> None:
     ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=2000, help="timed steps of the 512-cavity batch")
+    ap.add_argument("--batch-adjoint-steps", type=int, default=100, help="batched adjoint steps (0: skip)")
     ap.add_argument("--calibrate", action="store_true",
                     help="short copy-bandwidth run (3 copies) for rocprofv3 --pmc passes")
     a = ap.parse_args()
@@ -253,7 +261,7 @@ def main() -> None:
     if "f16" in res and "f32" in res:
         line["f16_over_f32"] = round(res["f16"]["mlups"] / res["f32"]["mlups"], 3)
     line["adjoint"] = None if a.no_adjoint else adjoint_leg(capi)
-    line["batch"] = None if a.no_batch else batch_leg(capi)
+    line["batch"] = None if a.no_batch else batch_leg(capi, a.batch_steps, adj_steps=a.batch_adjoint_steps)
     line["cpu_baseline"] = None if a.no_cpu else cpu_baseline(rho, u, a.cpu_steps)
     print(json.dumps(line), flush=True)
 

@@ -73,7 +73,9 @@ namespace {
 
 int make_plan(lbm_domain* d) {
     const ModeOps* o = d->ops;
-    int V = d->tune_vec ? d->tune_vec : o->default_vec;
+    if (d->compact && d->tune_vec > 1)
+        return fail(LBM_ERR_ARG, "this domain packs its rows (short x rows): only 1 cell per thread");
+    int V = d->tune_vec ? d->tune_vec : (d->compact ? o->vecs[0] : o->default_vec);
     bool ok = false;
     for (int i = 0; i < 4 && o->vecs[i]; ++i) ok |= (o->vecs[i] == V);
     if (!ok) return fail(LBM_ERR_ARG, "vector width " + std::to_string(V) + " not built for this mode");
@@ -98,8 +100,10 @@ int make_plan(lbm_domain* d) {
     p.tpp = static_cast<int>((plane_cells + 63) / 64);
     p.nblocks_cells = (static_cast<int64_t>(p.tpp) * d->L.zl + 3) / 4;
     p.rxl = 1.0 / static_cast<double>(d->L.xl);
-    p.cells = d->tune_vec == 0 && d->nb > 0 && p.nblocks_cells <= 0x7fffffffLL &&
-              (plane_cells + 63) / 64 <= 0x7fffffffLL;
+    p.cells = d->tune_vec == 0 && (d->nb > 0 || d->compact) &&
+              p.nblocks_cells <= 0x7fffffffLL && (plane_cells + 63) / 64 <= 0x7fffffffLL;
+    if (!p.cells && d->compact && V > 1)
+        return fail(LBM_ERR_UNSUPPORTED, "packed rows need the cell-tile kernel or 1 cell per thread");
     return LBM_OK;
 }
 
@@ -199,8 +203,13 @@ int lbm_domain_create(const lbm_desc* desc, lbm_domain** out) {
     d->esize = elem_size(d->mode);
     const int64_t A = 128 / static_cast<int64_t>(d->esize);
     FieldLayout& L = d->L;
-    L.off = A - 1;
-    L.px = roundup(d->nx, A);
+    // Rows start on a 128-byte line (x = 1 aligned for the vector row
+    // kernel) unless a row is shorter than two lines: then padding would be a
+    // large share of every line read, and rows are packed at pitch xl + 2
+    // for the cell-tile kernel, which needs no alignment.
+    d->compact = d->nx < 2 * A;
+    L.off = d->compact ? 0 : A - 1;
+    L.px = d->compact ? d->nx : roundup(d->nx, A);
     L.pxy = L.px * d->ny;
     // Planes are A-aligned; LBM_PLANE_ALIGN (elements) rounds them further
     // (a tuning knob for HBM channel mapping of the Q concurrent streams).

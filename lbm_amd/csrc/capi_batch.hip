@@ -30,6 +30,7 @@ struct lbm_batch {
     hipStream_t stream = nullptr;
     int V = 0, bpc = 8, bx_policy = 0;
     int kernel = BK_CELLS;      // enum BatchKernel
+    bool any_compact = false;   // some domain packs its rows: row kernels take V = 1 only
     int64_t near_cells = 0;     // NEAR cells of all domains
     BatchView* d_views = nullptr;
     TileRef* d_tiles = nullptr;
@@ -67,6 +68,9 @@ void free_lists(lbm_batch* b) {
 // every plane of every domain in 64-cell pieces.
 int configure(lbm_batch* b, int V, int bx_policy, int bpc, int kernel) {
     if (kernel == BK_CELLS) V = 1;
+    if (V > 1)
+        for (const lbm_domain* d : b->doms)
+            if (d->compact) return fail(LBM_ERR_ARG, "a domain with packed rows takes 1 cell per thread");
     drop_graph(b);
     std::vector<BatchView> views;
     std::vector<TileRef> tiles;
@@ -206,8 +210,10 @@ int lbm_batch_create(lbm_domain* const* domains, int n, lbm_batch** out) {
     b->device = d0->desc.device;
     (void)hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device);
     if (b->ncu <= 0) b->ncu = 256;
-    for (const lbm_domain* d : b->doms)
+    for (const lbm_domain* d : b->doms) {
         b->ncells += static_cast<uint64_t>(d->L.xl) * d->L.yl * d->L.zl;
+        b->any_compact |= d->compact;
+    }
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         delete b;
         return fail(LBM_ERR_HIP, "hipStreamCreate failed");
@@ -352,6 +358,7 @@ int lbm_batch_tune(lbm_batch* b, int trial_steps, double* best_ms) {
                 for (int bpc : {0, 4, 8, 16}) {
                     // cell tiles have one shape: try it once per grid size
                     if (kern == BK_CELLS && (i > 0 || pol > 0)) continue;
+                    if (kern != BK_CELLS && b->ops->vecs[i] > 1 && b->any_compact) continue;
                     if (int rc = configure(b, b->ops->vecs[i], pol, bpc, kern)) return rc;
                     // trial runs advance the fields; tuning happens before the timed run
                     int p, m;

@@ -89,6 +89,7 @@ struct lbm_domain {
     int nx = 0, ny = 0, nz = 0;
     int64_t ncell = 0;
     FieldLayout L{};
+    bool compact = false;  // rows packed at pitch xl + 2 (no line alignment)
     size_t esize = 0;
     size_t field_bytes = 0;
     void* buf[2] = {nullptr, nullptr};

@@ -44,7 +44,11 @@ def test_batch_equals_single(storage, config):
     batch_doms = [cavity(*s[:4], storage) for s in specs]
     single_doms = [cavity(*s[:4], storage) for s in specs]
     b = capi.Batch(batch_doms)
-    vec = 2 if storage != "f64s" else 1
+    # these cavities are short in x, so their rows are packed and the row
+    # kernels take one cell per thread (not built for 16-bit storage)
+    vec = 1
+    if config.startswith("rows") and storage == "f16":
+        pytest.skip("row kernels of 16-bit storage need rows of at least 126 cells")
     if config == "tuned":
         say(f"  tune: {b.tune(2):.3f} ms/step, {b.info()}")
     else:
