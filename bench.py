@@ -223,6 +223,9 @@ def main() -> None:
     # plain 1 GiB device copy: achievable-bandwidth reference, and the kernel of
     # known size that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters
     copy = capi.copy_bandwidth(1 << 30, reps=3 if a.calibrate else 10)
+    # the same data movement as a step (Q planes read, Q planes written)
+    # without the arithmetic or the pull offsets
+    qcopy = capi.copy_planes_bandwidth(Q, N ** 3, reps=3 if a.calibrate else 10)
     rho, u = tgv(N)
     modes = [m for m in a.modes.split(",") if m]
     res = {m: run_mode(capi, m, a.steps, a.warmup, rho, u) for m in modes}
@@ -252,7 +255,9 @@ def main() -> None:
                      "kernel": "k_bulk", "kernel_ms": round(head["kernel_ms"], 5),
                      "alg_bytes_per_launch": head["alg_bytes"],
                      "copy_gbs": round(copy["gbs"], 1),
-                     "frac_of_copy": round(head["achieved_gbs"] / copy["gbs"], 4)},
+                     "frac_of_copy": round(head["achieved_gbs"] / copy["gbs"], 4),
+                     "q_plane_copy_gbs": round(qcopy["gbs"], 1),
+                     "frac_of_q_plane_copy": round(head["achieved_gbs"] / qcopy["gbs"], 4)},
         "modes": {m: {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
                       "achieved_gbs": round(r["achieved_gbs"], 1),
                       "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),

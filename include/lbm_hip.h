@@ -159,6 +159,11 @@ int lbm_time_steps(lbm_domain* d, uint64_t nsteps, double out[4]);
  * Used to calibrate HBM traffic counters and as the achievable-bandwidth
  * reference next to the roofline. */
 int lbm_copy_bandwidth(int device, uint64_t bytes, int reps, double out[2]);
+/* Extension: bandwidth of a q-plane fp32 copy (q <= 27 planes of plane_elems
+ * elements, even; each thread loads its two elements of every plane, then
+ * stores them non-temporally): the q read + q write streams of one step
+ * without its arithmetic. out as lbm_copy_bandwidth. */
+int lbm_copy_planes_bandwidth(int device, int q, uint64_t plane_elems, int reps, double out[2]);
 
 /* ---------------------------------------------------------------------------
  * Differentiation (extension; the reference has none). A tape records one

@@ -69,6 +69,7 @@ SIGNATURES = [
     ("lbm_conservation", c_int, [c_void_p, POINTER(c_double)]),
     ("lbm_time_steps", c_int, [c_void_p, c_uint64, POINTER(c_double)]),
     ("lbm_copy_bandwidth", c_int, [c_int, c_uint64, c_int, POINTER(c_double)]),
+    ("lbm_copy_planes_bandwidth", c_int, [c_int, c_int, c_uint64, c_int, POINTER(c_double)]),
     ("lbm_adj_create", c_int, [c_void_p, c_void_p, POINTER(c_void_p)]),
     ("lbm_adj_destroy", c_int, [c_void_p]),
     ("lbm_adj_forward", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
@@ -131,6 +132,13 @@ def copy_bandwidth(nbytes: int, reps: int = 10, device: int = 0) -> dict:
     out = np.zeros(2)
     _check(load().lbm_copy_bandwidth(device, nbytes, reps, out.ctypes.data_as(POINTER(c_double))))
     return {"ms": float(out[0]), "gbs": float(out[1]), "bytes": int(nbytes // 16 * 16)}
+
+
+def copy_planes_bandwidth(q: int, plane_elems: int, reps: int = 10, device: int = 0) -> dict:
+    """q-plane fp32 copy (lbm_copy_planes_bandwidth): ms per copy and GB/s."""
+    out = np.zeros(2)
+    _check(load().lbm_copy_planes_bandwidth(device, q, plane_elems, reps, out.ctypes.data_as(POINTER(c_double))))
+    return {"ms": float(out[0]), "gbs": float(out[1]), "q": q, "plane_elems": int(plane_elems)}
 
 
 def _dptr(a: np.ndarray):

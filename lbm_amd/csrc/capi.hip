@@ -59,6 +59,26 @@ __global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, u
     if (i < n) dst[i] = src[i];
 }
 
+// q-plane copy: thread i copies elements 2i, 2i + 1 of every plane (all
+// loads, then all non-temporal stores): the data movement of one pull step
+// without its arithmetic, as a bandwidth reference for q read and q write
+// streams.
+template <int Q>
+__global__ void __launch_bounds__(256) k_copy_planes(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                     int64_t plane2) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= plane2) return;
+    float2 v[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) v[k] = src[k * plane2 + i];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        uint32_t __attribute__((ext_vector_type(2))) r;
+        __builtin_memcpy(&r, &v[k], sizeof(r));
+        __builtin_nontemporal_store(r, reinterpret_cast<uint32_t __attribute__((ext_vector_type(2)))*>(dst + k * plane2 + i));
+    }
+}
+
 template <class T>
 __global__ void k_fill_planes(T* buf, int64_t plane, int q, const double* values) {
     const int64_t n = plane * q;
@@ -640,6 +660,55 @@ int lbm_copy_bandwidth(int device, uint64_t bytes, int reps, double out[2]) {
     if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("copy: ") + hipGetErrorString(e));
     out[0] = ms / reps;
     out[1] = 2.0 * static_cast<double>(n) * 16.0 / (out[0] * 1e-3) / 1e9;
+    return LBM_OK;
+}
+
+int lbm_copy_planes_bandwidth(int device, int q, uint64_t plane_elems, int reps, double out[2]) {
+    if (!out || (q != 15 && q != 19 && q != 27) || plane_elems < 2 || plane_elems % 2 || reps < 1 || reps > 1000)
+        return fail(LBM_ERR_ARG, "bad plane copy arguments (q = 15, 19 or 27, even plane size)");
+    auto launch = [&](const void* s, void* d, int64_t p2, uint32_t g) {
+        if (q == 15)
+            hipLaunchKernelGGL(k_copy_planes<15>, dim3(g), dim3(256), 0, nullptr, static_cast<const float2*>(s),
+                               static_cast<float2*>(d), p2);
+        else if (q == 19)
+            hipLaunchKernelGGL(k_copy_planes<19>, dim3(g), dim3(256), 0, nullptr, static_cast<const float2*>(s),
+                               static_cast<float2*>(d), p2);
+        else
+            hipLaunchKernelGGL(k_copy_planes<27>, dim3(g), dim3(256), 0, nullptr, static_cast<const float2*>(s),
+                               static_cast<float2*>(d), p2);
+    };
+    HIPCHK(hipSetDevice(device));
+    const int64_t plane2 = static_cast<int64_t>(plane_elems / 2);
+    const size_t bytes = static_cast<size_t>(plane_elems) * 4 * q;
+    void* a = nullptr;
+    void* b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return fail(LBM_ERR_ALLOC, "hipMalloc failed");
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(LBM_ERR_ALLOC, "hipMalloc failed");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const uint32_t grid = static_cast<uint32_t>((plane2 + 255) / 256);
+    if (e == hipSuccess) {
+        launch(a, b, plane2, grid);
+        e = hipEventRecord(e0, nullptr);
+    }
+    for (int r = 0; e == hipSuccess && r < reps; ++r) launch(a, b, plane2, grid);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("plane copy: ") + hipGetErrorString(e));
+    out[0] = ms / reps;
+    out[1] = 2.0 * static_cast<double>(bytes) / (out[0] * 1e-3) / 1e9;
     return LBM_OK;
 }
 
