@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define LBM_HIP_ABI_VERSION 2
+#define LBM_HIP_ABI_VERSION 3
 
 /* Status codes. */
 #define LBM_OK 0
@@ -64,6 +64,15 @@ enum lbm_bc {
     LBM_BC_PRESSURE = 6
 };
 
+/* Distribution storage: two buffers stepped A -> B -> A (the reference's
+ * two fields), or one buffer updated in place with the AA pattern (half the
+ * memory; NoSlip, MovingWall, Inflow, Outflow and Pressure boundaries; no
+ * adjoint or batching). Both give the same bits in fp64 and fp32 storage. */
+enum lbm_layout {
+    LBM_LAYOUT_AB = 0,
+    LBM_LAYOUT_AA = 1
+};
+
 typedef struct lbm_desc {
     int q;             /* 15, 19 or 27: lbm::model::d3q15/d3q19/d3q27 (include/model.h) */
     int storage;       /* enum lbm_storage */
@@ -73,6 +82,7 @@ typedef struct lbm_desc {
     double tau;        /* BGKCollision(tau), include/collision.hpp:55-58 */
     double origin[3];  /* Domain origin (VTK output only) */
     double spacing[3]; /* Domain spacing (VTK output only) */
+    int layout;        /* enum lbm_layout */
 } lbm_desc;
 
 typedef struct lbm_domain lbm_domain;
@@ -296,6 +306,7 @@ typedef struct lbm_info {
     int block_x, block_y;      /* bulk kernel block shape */
     int nconds;                /* distinct boundary conditions */
     int cell_tiles;            /* 1: steps run as 64-cell tiles per wavefront (k_cells) */
+    int layout;                /* enum lbm_layout */
 } lbm_info;
 int lbm_get_info(lbm_domain* d, lbm_info* info);
 

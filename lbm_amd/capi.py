@@ -38,14 +38,15 @@ class LbmError(RuntimeError):
 class Desc(ctypes.Structure):
     _fields_ = [("q", c_int), ("storage", c_int), ("device", c_int), ("periodic", c_int * 3),
                 ("xl", c_uint64), ("yl", c_uint64), ("zl", c_uint64), ("tau", c_double),
-                ("origin", c_double * 3), ("spacing", c_double * 3)]
+                ("origin", c_double * 3), ("spacing", c_double * 3), ("layout", c_int)]
 
 
 class Info(ctypes.Structure):
     _fields_ = [("ncells_interior", c_uint64), ("ncells_fluid", c_uint64),
                 ("ncells_near", c_uint64), ("bytes_per_value", c_uint64),
                 ("field_bytes", c_uint64), ("device_bytes", c_uint64), ("vec", c_int),
-                ("block_x", c_int), ("block_y", c_int), ("nconds", c_int), ("cell_tiles", c_int)]
+                ("block_x", c_int), ("block_y", c_int), ("nconds", c_int), ("cell_tiles", c_int),
+                ("layout", c_int)]
 
 
 # (name, restype, argtypes) of every exported entry point in include/lbm_hip.h
@@ -150,11 +151,12 @@ class Domain:
 
     def __init__(self, xl: int, yl: int, zl: int, tau: float, q: int = 19, storage: str = "f64",
                  periodic=(False, False, False), device: int = 0,
-                 origin=(0.0, 0.0, 0.0), spacing=(1.0, 1.0, 1.0)):
+                 origin=(0.0, 0.0, 0.0), spacing=(1.0, 1.0, 1.0), layout: str = "ab"):
         lib = load()
         d = Desc()
         _check(lib.lbm_desc_init(byref(d)))
         d.q, d.storage, d.device = q, STORAGE[storage], device
+        d.layout = {"ab": 0, "aa": 1}[layout]
         for k in range(3):
             d.periodic[k] = 1 if periodic[k] else 0
             d.origin[k] = origin[k]

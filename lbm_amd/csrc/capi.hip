@@ -106,6 +106,8 @@ int make_plan(lbm_domain* d) {
     if (!bx) {
         bx = 1;
         while (bx < nvx && bx < 256) bx *= 2;
+        // the mode's preferred width, where rows are at least that long
+        if (o->default_bx && d->tune_vec == 0 && bx > o->default_bx) bx = o->default_bx;
     }
     if (bx < 1 || bx > 256 || (256 % bx)) return fail(LBM_ERR_ARG, "block_x must divide 256");
     p.BX = bx;
@@ -197,6 +199,7 @@ int lbm_domain_create(const lbm_desc* desc, lbm_domain** out) {
     if (desc->q != 15 && desc->q != 19 && desc->q != 27)
         return fail(LBM_ERR_ARG, "q must be 15, 19 or 27");
     if (desc->storage < 0 || desc->storage > 4) return fail(LBM_ERR_ARG, "unknown storage mode");
+    if (desc->layout != LBM_LAYOUT_AB && desc->layout != LBM_LAYOUT_AA) return fail(LBM_ERR_ARG, "unknown layout");
     const uint64_t maxl = 1u << 20;
     if (desc->xl < 1 || desc->yl < 1 || desc->zl < 1 || desc->xl > maxl || desc->yl > maxl ||
         desc->zl > maxl)
@@ -254,13 +257,15 @@ int lbm_domain_create(const lbm_desc* desc, lbm_domain** out) {
     };
     if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(LBM_ERR_HIP, "hipStreamCreate failed");
-    for (int k = 0; k < 2; ++k)
+    d->aa = desc->layout == LBM_LAYOUT_AA;
+    const int nbuf = d->aa ? 1 : 2;
+    for (int k = 0; k < nbuf; ++k)
         if (hipMalloc(&d->buf[k], d->field_bytes) != hipSuccess)
             return bail(LBM_ERR_ALLOC, "hipMalloc of a distribution buffer failed");
     if (hipMalloc(&d->d_kind, static_cast<size_t>(L.plane)) != hipSuccess)
         return bail(LBM_ERR_ALLOC, "hipMalloc of the kind array failed");
-    d->device_bytes = 2 * d->field_bytes + static_cast<size_t>(L.plane);
-    for (int k = 0; k < 2; ++k) {
+    d->device_bytes = nbuf * d->field_bytes + static_cast<size_t>(L.plane);
+    for (int k = 0; k < nbuf; ++k) {
         int rc = fill_initial(d, d->buf[k]);
         if (rc) return bail(rc, last_error());
     }
@@ -344,6 +349,10 @@ int lbm_set_brinkman(lbm_domain* d, const double* alpha, const double g[3]) {
 int lbm_finalize(lbm_domain* d) {
     if (int rc = check_handle(d)) return rc;
     if (d->finalized) return fail(LBM_ERR_STATE, "already finalized");
+    if (d->aa)
+        for (const HostCond& c : d->conds)
+            if (c.kind == LBM_BC_FREESLIP)
+                return fail(LBM_ERR_UNSUPPORTED, "the in-place layout has no FreeSlip boundary (use the two-buffer layout)");
     const HostLattice& lat = d->lat;
     const int Q = d->Q;
     const FieldLayout& L = d->L;
@@ -484,6 +493,7 @@ int lbm_finalize(lbm_domain* d) {
 
 int lbm_set_tuning(lbm_domain* d, int vec, int block_x) {
     if (!d) return fail(LBM_ERR_ARG, "null domain handle");
+    if (d->aa && vec > 0) return fail(LBM_ERR_UNSUPPORTED, "the in-place layout steps with cell tiles only");
     const int ov = d->tune_vec, ob = d->tune_bx;
     d->tune_vec = vec;
     d->tune_bx = block_x;
@@ -499,6 +509,23 @@ int lbm_set_tuning(lbm_domain* d, int vec, int block_x) {
 }
 
 static int run_steps(lbm_domain* d, uint64_t nsteps, hipEvent_t* ev) {
+    if (d->aa) {
+        for (uint64_t t = 0; t < nsteps; ++t) {
+            const StepArgs a = d->args(d->buf[0], d->buf[0], d->materialised ? 1 : 0);
+            hipEvent_t* e4 = ev ? ev + 4 * t : nullptr;
+            if (e4) (void)hipEventRecord(e4[0], d->stream);
+            hipError_t e = d->ops->aa_step(a, d->plan, d->aa_phase, d->stream);
+            if (e4) {
+                (void)hipEventRecord(e4[1], d->stream);
+                (void)hipEventRecord(e4[2], d->stream);
+                (void)hipEventRecord(e4[3], d->stream);
+            }
+            if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e));
+            d->aa_phase ^= 1;
+            d->materialised = false;
+        }
+        return LBM_OK;
+    }
     for (uint64_t t = 0; t < nsteps; ++t) {
         const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur ^ 1], d->materialised ? 1 : 0);
         hipError_t e = d->ops->step(a, d->plan, d->stream, ev ? ev + 4 * t : nullptr);
@@ -506,6 +533,17 @@ static int run_steps(lbm_domain* d, uint64_t nsteps, hipEvent_t* ev) {
         d->cur ^= 1;
         d->materialised = false;
     }
+    return LBM_OK;
+}
+
+// In-place layout: bring the field to the natural representation (the
+// readers' and writers' layout) if the last step left it swapped.
+static int aa_settle(lbm_domain* d) {
+    if (!d->aa || d->aa_phase == 0) return LBM_OK;
+    const StepArgs a = d->args(d->buf[0], d->buf[0], 0);
+    const hipError_t e = d->ops->aa_swap(a, d->plan, d->stream);
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("in-place layout swap: ") + hipGetErrorString(e));
+    d->aa_phase = 0;
     return LBM_OK;
 }
 
@@ -552,7 +590,9 @@ int lbm_get_pdf(lbm_domain* d, double* out) {
     if (int rc = check_handle(d)) return rc;
     if (!out) return fail(LBM_ERR_ARG, "null output");
     if (!d->finalized && !d->materialised) return fail(LBM_ERR_STATE, "inconsistent state");
-    if (d->finalized && !d->materialised) {
+    if (int rc = aa_settle(d)) return rc;
+    // (in the in-place layout the boundary cells already hold the rule values)
+    if (d->finalized && !d->materialised && !d->aa) {
         const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur], 0);
         HIPCHK(d->ops->materialise(a, d->nx, d->ny, d->nz, d->stream));
     }
@@ -579,6 +619,7 @@ int lbm_set_pdf(lbm_domain* d, const double* in) {
     (void)hipFree(dev);
     if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("set_pdf: ") + hipGetErrorString(e));
     d->materialised = true;
+    d->aa_phase = 0;
     return LBM_OK;
 }
 
@@ -587,6 +628,11 @@ int lbm_init_equilibrium(lbm_domain* d, const double* rho, const double* u) {
     if (!d->finalized) return fail(LBM_ERR_STATE, "call lbm_finalize first (fluid cells are needed)");
     if (!rho || !u) return fail(LBM_ERR_ARG, "null input");
     const size_t n = static_cast<size_t>(d->L.xl) * d->L.yl * d->L.zl;
+    if (d->aa) {
+        // boundary cells back to the weights, as the two-buffer layout holds them
+        if (int rc = fill_initial(d, d->buf[0])) return rc;
+        d->aa_phase = 0;
+    }
     double* dev = nullptr;
     HIPCHK(hipMalloc(&dev, 4 * n * sizeof(double)));
     hipError_t e = hipMemcpyAsync(dev, rho, n * sizeof(double), hipMemcpyHostToDevice, d->stream);
@@ -601,6 +647,7 @@ int lbm_init_equilibrium(lbm_domain* d, const double* rho, const double* u) {
 
 int lbm_moments(lbm_domain* d, double* rho, double* u) {
     if (int rc = check_handle(d)) return rc;
+    if (int rc = aa_settle(d)) return rc;
     const size_t n = static_cast<size_t>(d->L.xl) * d->L.yl * d->L.zl;
     double* dev = nullptr;
     HIPCHK(hipMalloc(&dev, 4 * n * sizeof(double)));
@@ -617,6 +664,7 @@ int lbm_conservation(lbm_domain* d, double out[4]) {
     if (int rc = check_handle(d)) return rc;
     if (!d->finalized) return fail(LBM_ERR_STATE, "call lbm_finalize first");
     if (!out) return fail(LBM_ERR_ARG, "null output");
+    if (int rc = aa_settle(d)) return rc;
     HIPCHK(d->ops->conservation(d->L, d->buf[d->cur], d->d_kind, d->d_partial, d->nred, d->d_out4,
                                 d->stream));
     HIPCHK(hipMemcpyAsync(out, d->d_out4, 4 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
@@ -725,7 +773,8 @@ int lbm_get_info(lbm_domain* d, lbm_info* info) {
     info->block_x = d->plan.BX;
     info->block_y = d->plan.BY;
     info->nconds = d->nconds_dev;
-    info->cell_tiles = d->plan.cells ? 1 : 0;
+    info->cell_tiles = (d->plan.cells || d->aa) ? 1 : 0;
+    info->layout = d->aa ? LBM_LAYOUT_AA : LBM_LAYOUT_AB;
     return LBM_OK;
 }
 

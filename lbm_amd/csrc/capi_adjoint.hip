@@ -91,6 +91,7 @@ int lbm_adj_create(lbm_domain* d, const lbm_adj_desc* desc, lbm_tape** out) {
     if (!desc || !out) return fail(LBM_ERR_ARG, "null argument");
     *out = nullptr;
     if (!d->finalized) return fail(LBM_ERR_STATE, "finalize the domain before creating a tape");
+    if (d->aa) return fail(LBM_ERR_UNSUPPORTED, "the adjoint needs the two-buffer layout");
     const AdjOps* ops = adj_ops(d->Q, d->mode);
     if (!ops) return fail(LBM_ERR_UNSUPPORTED, "adjoint needs D3Q19 with F32 or F64S storage");
     for (const HostCond& c : d->conds)

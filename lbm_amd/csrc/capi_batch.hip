@@ -201,6 +201,7 @@ int lbm_batch_create(lbm_domain* const* domains, int n, lbm_batch** out) {
         if (d->Q != d0->Q || d->mode != d0->mode || d->desc.device != d0->desc.device)
             return fail(LBM_ERR_ARG, "batched domains must share lattice, storage and device");
         if (d->d_alpha) return fail(LBM_ERR_UNSUPPORTED, "batched Brinkman domains are not supported");
+        if (d->aa) return fail(LBM_ERR_UNSUPPORTED, "batching needs the two-buffer layout");
     }
     const BatchOps* ops = batch_ops(d0->Q, d0->mode);
     if (!ops) return fail(LBM_ERR_UNSUPPORTED, "batching needs D3Q19 with F32, F16, BF16 or F64S storage");

@@ -90,6 +90,8 @@ struct lbm_domain {
     int64_t ncell = 0;
     FieldLayout L{};
     bool compact = false;  // rows packed at pitch xl + 2 (no line alignment)
+    bool aa = false;       // one buffer updated in place (AA pattern), buf[0] only
+    int aa_phase = 0;      // AA: 0 natural representation, 1 swapped
     size_t esize = 0;
     size_t field_bytes = 0;
     void* buf[2] = {nullptr, nullptr};

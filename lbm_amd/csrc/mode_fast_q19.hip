@@ -6,7 +6,8 @@ namespace lbm_amd {
 
 namespace {
 using Lat = D3Q19;
-const ModeOps kF32 = OpsImpl<Lat, Mode::F32>::make<1, 2, 4>(2);
+// fp32: 32 x 8 thread blocks (measured 1 % faster than full-row blocks)
+const ModeOps kF32 = OpsImpl<Lat, Mode::F32>::make<1, 2, 4>(2, 32);
 const ModeOps kF16 = OpsImpl<Lat, Mode::F16>::make<2, 4, 8>(4);
 const ModeOps kBF16 = OpsImpl<Lat, Mode::BF16>::make<2, 4, 8>(4);
 const ModeOps kF64S = OpsImpl<Lat, Mode::F64S>::make<1, 2>(1);

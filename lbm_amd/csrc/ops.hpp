@@ -34,6 +34,7 @@ struct ModeOps {
     Mode mode;
     int q;
     int default_vec;
+    int default_bx;  // threads along x of a k_bulk block when not tuned (0: row width)
     // Supported values of StepPlan::V, 0-terminated.
     int vecs[4];
     // bulk + near for one step (src/dst in args). ev (may be null) receives
@@ -51,6 +52,10 @@ struct ModeOps {
     hipError_t (*init_eq)(const FieldLayout& Ly, void* dst, const uint8_t* kind,
                           const double* rho, const double* u, hipStream_t s);
     hipError_t (*materialise)(const StepArgs& a, int nx, int ny, int nz, hipStream_t s);
+    // in-place (AA) layout: one step from representation phase (0: natural,
+    // 1: swapped) to the other; and the swapped -> natural conversion
+    hipError_t (*aa_step)(const StepArgs& a, const StepPlan& p, int phase, hipStream_t s);
+    hipError_t (*aa_swap)(const StepArgs& a, const StepPlan& p, hipStream_t s);
 };
 
 // Defined in the mode translation units.

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "kernels.hpp"
+#include "kernels_aa.hpp"
 #include "kernels_fields.hpp"
 #include "ops.hpp"
 
@@ -110,12 +111,27 @@ struct OpsImpl {
         return hipGetLastError();
     }
 
+    static hipError_t aa_step(const StepArgs& a, const StepPlan& p, int phase, hipStream_t s) {
+        const dim3 grid(static_cast<uint32_t>(p.nblocks_cells));
+        if (phase == 0)
+            hipLaunchKernelGGL((k_aa_eo<L, M>), grid, dim3(256), 0, s, a, p.tpp, p.rxl);
+        else
+            hipLaunchKernelGGL((k_aa_oe<L, M>), grid, dim3(256), 0, s, a, p.tpp, p.rxl);
+        return hipGetLastError();
+    }
+    static hipError_t aa_swap(const StepArgs& a, const StepPlan& p, hipStream_t s) {
+        hipLaunchKernelGGL((k_aa_swap<L, M>), dim3(static_cast<uint32_t>(p.nblocks_cells)), dim3(256), 0, s, a,
+                           p.tpp, p.rxl);
+        return hipGetLastError();
+    }
+
     template <int V0, int... Vs>
-    static ModeOps make(int default_vec) {
+    static ModeOps make(int default_vec, int default_bx = 0) {
         ModeOps o{};
         o.mode = M;
         o.q = L::Q;
         o.default_vec = default_vec;
+        o.default_bx = default_bx;
         const int v[] = {V0, Vs...};
         for (int i = 0; i < 4; ++i) o.vecs[i] = i < static_cast<int>(sizeof(v) / sizeof(int)) ? v[i] : 0;
         o.step = &step_impl<V0, Vs...>;
@@ -125,6 +141,8 @@ struct OpsImpl {
         o.conservation = &conservation;
         o.init_eq = &init_eq;
         o.materialise = &materialise;
+        o.aa_step = &aa_step;
+        o.aa_swap = &aa_swap;
         return o;
     }
 };
