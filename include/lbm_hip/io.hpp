@@ -43,6 +43,7 @@ class Config {
     uint32_t _omp_threads{1};
     std::string _lattice{"D3Q19"};
     std::string _storage{"f64"};
+    std::string _layout{"ab"};
 
     static std::string trim(const std::string& s) {
         size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
@@ -56,7 +57,7 @@ class Config {
     static bool known(const std::string& k) {
         static const char* keys[] = {"collision-model", "tau", "timesteps", "timesteps-per-plot",
                                      "scenario-file", "omp-threads", "output-dir", "lattice",
-                                     "storage"};
+                                     "storage", "layout"};
         for (const char* s : keys)
             if (k == s) return true;
         return false;
@@ -72,6 +73,7 @@ class Config {
             else if (k == "output-dir") _output_dir = v;
             else if (k == "lattice") _lattice = v;
             else if (k == "storage") _storage = v;
+            else if (k == "layout") _layout = v;
         } catch (const std::exception&) {
             throw std::logic_error("the argument ('" + v + "') for option '--" + k + "' is invalid");
         }
@@ -111,7 +113,7 @@ public:
             std::cout << "Usage: lbm [file] additional-options...\n"
                       << "  --collision-model bgk --tau T --timesteps N --timesteps-per-plot P\n"
                       << "  --scenario-file F --omp-threads N --output-dir D\n"
-                      << "  --lattice D3Q15|D3Q19|D3Q27 --storage f64|f32|f16|bf16|f64s" << std::endl;
+                      << "  --lattice D3Q15|D3Q19|D3Q27 --storage f64|f32|f16|bf16|f64s --layout ab|aa" << std::endl;
             std::exit(1);  // as configuration.h:109-114
         }
         std::map<std::string, std::string> file;
@@ -169,6 +171,11 @@ public:
     std::string scenario_xml() const { return _scenario_xml; }
     std::string lattice() const { return _lattice; }
     std::string storage() const { return _storage; }
+    int layout_id() const {
+        if (_layout == "ab") return LBM_LAYOUT_AB;
+        if (_layout == "aa") return LBM_LAYOUT_AA;
+        throw std::logic_error("layout must be ab or aa");
+    }
     int storage_id() const {
         if (_storage == "f64") return LBM_STORAGE_F64;
         if (_storage == "f32") return LBM_STORAGE_F32;
@@ -416,6 +423,7 @@ Domain_ptr<M> parse_scenario_file(const std::string& filename, Config& cfg,
                                   FluidCollision<M>& collision) {
     DeviceOptions opt;
     opt.storage = cfg.storage_id();
+    opt.layout = cfg.layout_id();
     return parse_scenario_file<M>(filename, cfg, collision, opt);
 }
 

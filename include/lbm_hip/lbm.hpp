@@ -72,6 +72,7 @@ struct DeviceOptions {
     int storage = LBM_STORAGE_F64;  // the reference's fp64 arithmetic by default
     int device = 0;
     std::array<int, 3> periodic{{0, 0, 0}};
+    int layout = LBM_LAYOUT_AB;  // LBM_LAYOUT_AA: one buffer updated in place
 };
 
 template <class M>
@@ -236,6 +237,7 @@ public:
         d.storage = opt.storage;
         d.device = opt.device;
         for (int k = 0; k < 3; ++k) d.periodic[k] = opt.periodic[k];
+        d.layout = opt.layout;
         d.xl = xl;
         d.yl = yl;
         d.zl = zl;

@@ -28,6 +28,7 @@ int run(lbm::io::Config& cfg) {
     auto collision = lbm::BGKCollision<model>(cfg.tau());
     lbm::DeviceOptions opt;
     opt.storage = cfg.storage_id();
+    opt.layout = cfg.layout_id();
     auto domain = lbm::io::parse_scenario_file<model>(cfg.scenario_xml(), cfg, collision, opt);
     std::cout << cfg << std::endl;
     std::cout << "> Domain lengths (x,y,z): " << domain->xlength() << ", " << domain->ylength()

@@ -90,6 +90,22 @@ def test_driver_matches_oracle(tmp_path, name, lattice, q):
 
 
 @pytest.mark.gpu
+def test_driver_inplace_layout(tmp_path):
+    """`layout = aa` (one buffer updated in place) gives the oracle's bits too."""
+    print("driver step.xml layout aa", flush=True)
+    scen = GOLDEN / "step.xml"
+    steps = 17
+    cfg = write_cfg(tmp_path, scen, steps, steps, "layout = aa\n")
+    r = run([cfg], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    sc = scenario.parse_scenario_file(scen)
+    rho, u = read_vts(tmp_path / "vtk" / f"{sc.name}.{steps}.vts")
+    ro, uo, fluid = oracle_moments(sc, steps, q=19)
+    assert np.array_equal(rho[fluid], ro[fluid])
+    assert np.array_equal(u[fluid], uo[fluid])
+
+
+@pytest.mark.gpu
 def test_driver_vtk_mask_scenario(tmp_path):
     """A pipe through a fluid mask (legacy STRUCTURED_POINTS file) with inflow
     and outflow, as build/scenarios/pipe.xml sets up."""
