@@ -124,8 +124,6 @@ __device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int
             any |= act[i];
         }
     }
-    if (!any) return;
-
     const int ym = wrap1(y - 1, Ly.yl), yp = wrap1(y + 1, Ly.yl);
     const int zm = wrap1(z - 1, Ly.zl), zp = wrap1(z + 1, Ly.zl);
     const int xm = wrap1(x0 - 1, Ly.xl);  // source of lane 0 for cx = +1
@@ -170,6 +168,10 @@ __device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int
 #pragma unroll
         for (int i = 0; i < V; ++i) f[i][q] = IO::template to_calc<q>(v[i]);
     });
+
+    // the loads above are issued before this test (every row they touch lies
+    // inside the ghosted box), so the kind lookup costs no extra round trip
+    if (!any) return;
 
     Calc alpha[V];
     if constexpr (BRINK) {
@@ -474,10 +476,12 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
     const FieldLayout& Ly = a.L;
     const int64_t e = cell_elem(Ly, x, y, z);
     const uint8_t k = a.kind[e];
-    if (k >= KIND_COND0) return;
     const T* __restrict__ src = static_cast<const T*>(a.src);
     T* __restrict__ dst = static_cast<T*>(a.dst);
 
+    // Pulls are issued before the kind test (sources of any interior cell lie
+    // inside the ghosted box), so the kind lookup adds no round trip; a
+    // non-fluid cell discards them.
     Calc f[Q];
     // Away from the faces of periodic axes every source is a fixed element
     // offset c_x + c_y px + c_z pxy away; on those faces the source wraps.
@@ -502,6 +506,7 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
             f[q] = IO::template to_calc<q>(src[q * Ly.plane + cell_elem(Ly, sx, sy, sz)]);
         });
     }
+    if (k >= KIND_COND0) return;
     if (k == KIND_NEAR && !a.materialised) {
         const OwnSums<Calc> os = own_sums_at<L, M>(src, Ly.plane, e);
         auto own = [&](auto kc) {
