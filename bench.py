@@ -10,7 +10,8 @@ Prints one JSON line: MLUPS of the fp32 mode (interior cells x steps / s),
 the roofline of the dominant kernel (algorithmic bytes 2*Q*sizeof(storage)
 per cell over its HIP-event-timed average launch), the fp16/bf16 modes and a
 CPU baseline (the fp64 CPU oracle, OpenMP on the host cores, bounded sample).
-Two further legs ride on the same line:
+Further legs ride on the same line:
+- "inplace_aa": the same workload in the in-place (AA) layout, one buffer;
 - "adjoint": the inverse problem on a periodic 128^3 D3Q19 fp32 box with a
   Brinkman porosity field (seeded uniform 32^3, trilinear), body force 1e-5 x,
   2000 steps with sqrt(N) checkpoints, plane-velocity misfit objective: wall
@@ -97,6 +98,24 @@ def run_mode(capi, mode: str, steps: int, warmup: int, rho, u):
         "vec": info["vec"],
         "block": [info["block_x"], info["block_y"]],
     }
+
+
+def aa_leg(capi, mode: str, steps: int, rho, u):
+    """The same workload in the in-place (AA) layout: one distribution buffer."""
+    out = {}
+    for layout in ("ab", "aa"):
+        d = capi.Domain(N, N, N, TAU, q=Q, storage=mode, periodic=(True, True, True), layout=layout)
+        d.finalize()
+        d.init_equilibrium(rho, u)
+        d.step(10)
+        d.sync()
+        ev = d.time_steps(steps)
+        out[layout] = {"kernel_ms": round(ev["bulk_ms"] / steps, 5), "device_gb": round(d.info()["device_bytes"] / 1e9, 3)}
+        d.close()
+    return {"storage": mode, "steps": steps,
+            "mlups": round(N ** 3 / (out["aa"]["kernel_ms"] * 1e3), 1),
+            "kernel_ms": out["aa"]["kernel_ms"], "device_gb": out["aa"]["device_gb"],
+            "two_buffer_kernel_ms": out["ab"]["kernel_ms"], "two_buffer_device_gb": out["ab"]["device_gb"]}
 
 
 def cpu_baseline(rho, u, steps: int = 4):
@@ -208,6 +227,7 @@ def main() -> None:
     ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-aa", action="store_true", help="skip the in-place layout line")
     ap.add_argument("--batch-steps", type=int, default=2000, help="timed steps of the 512-cavity batch")
     ap.add_argument("--batch-adjoint-steps", type=int, default=100, help="batched adjoint steps (0: skip)")
     ap.add_argument("--calibrate", action="store_true",
@@ -265,6 +285,8 @@ def main() -> None:
     }
     if "f16" in res and "f32" in res:
         line["f16_over_f32"] = round(res["f16"]["mlups"] / res["f32"]["mlups"], 3)
+    if not a.no_aa:
+        line["inplace_aa"] = aa_leg(capi, head_mode, min(a.steps, 200), rho, u)
     line["adjoint"] = None if a.no_adjoint else adjoint_leg(capi)
     line["batch"] = None if a.no_batch else batch_leg(capi, a.batch_steps, adj_steps=a.batch_adjoint_steps)
     line["cpu_baseline"] = None if a.no_cpu else cpu_baseline(rho, u, a.cpu_steps)
