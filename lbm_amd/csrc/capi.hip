@@ -10,7 +10,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
-#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -36,8 +35,9 @@ namespace {
 
 // Reference equilibrium (include/collision.hpp:34-51) on the host, for the
 // constant Inflow table.
-#pragma clang fp contract(off)
 void host_feq(const HostLattice& lat, double density, const double* v, double* feq) {
+    // separate multiplies and adds, as the kernels round them
+#pragma clang fp contract(off)
     const double cs = kCS;
     for (int q = 0; q < lat.Q; ++q) {
         double c_dot_u = 0.0;
@@ -51,7 +51,6 @@ void host_feq(const HostLattice& lat, double density, const double* v, double* f
                   u_dot_u / (2 * cs * cs));
     }
 }
-#pragma clang fp contract(on)
 
 __global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                 int64_t n) {
@@ -153,14 +152,13 @@ int fill_initial(lbm_domain* d, void* buf) {
         HIPCHK(hipMemsetAsync(buf, 0, d->field_bytes, d->stream));
         return LBM_OK;
     }
-    double* dv = nullptr;
-    HIPCHK(hipMalloc(&dv, sizeof(double) * d->Q));
-    HIPCHK(hipMemcpyAsync(dv, d->lat.w, sizeof(double) * d->Q, hipMemcpyHostToDevice, d->stream));
+    DeviceTemp dv;
+    HIPCHK(hipMalloc(&dv.p, sizeof(double) * d->Q));
+    HIPCHK(hipMemcpyAsync(dv.p, d->lat.w, sizeof(double) * d->Q, hipMemcpyHostToDevice, d->stream));
     hipLaunchKernelGGL(k_fill_planes<double>, dim3(4096), dim3(256), 0, d->stream,
-                       static_cast<double*>(buf), d->L.plane, d->Q, dv);
+                       static_cast<double*>(buf), d->L.plane, d->Q, static_cast<const double*>(dv.p));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(d->stream));
-    HIPCHK(hipFree(dv));
     return LBM_OK;
 }
 
@@ -234,14 +232,9 @@ int lbm_domain_create(const lbm_desc* desc, lbm_domain** out) {
     L.off = d->compact ? 0 : A - 1;
     L.px = d->compact ? d->nx : roundup(d->nx, A);
     L.pxy = L.px * d->ny;
-    // Planes are A-aligned; LBM_PLANE_ALIGN (elements) rounds them further
-    // (a tuning knob for HBM channel mapping of the Q concurrent streams).
-    int64_t palign = A;
-    if (const char* s = std::getenv("LBM_PLANE_ALIGN")) {
-        const long long v = std::atoll(s);
-        if (v > 0 && v % A == 0) palign = v;
-    }
-    L.plane = roundup(L.off + L.pxy * d->nz + 16, palign);
+    // Planes start on a 128-byte line; 16 spare elements at the end keep the
+    // vector kernel's last row reads inside the plane.
+    L.plane = roundup(L.off + L.pxy * d->nz + 16, A);
     L.xl = static_cast<int>(desc->xl);
     L.yl = static_cast<int>(desc->yl);
     L.zl = static_cast<int>(desc->zl);
@@ -563,27 +556,24 @@ int lbm_time_steps(lbm_domain* d, uint64_t nsteps, double out[4]) {
     if (int rc = check_handle(d)) return rc;
     if (!d->finalized) return fail(LBM_ERR_STATE, "call lbm_finalize first");
     if (!out || nsteps == 0 || nsteps > 100000) return fail(LBM_ERR_ARG, "nsteps in [1, 1e5]");
-    std::vector<hipEvent_t> ev(4 * nsteps, nullptr);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    int rc = run_steps(d, nsteps, ev.data());
-    if (rc == LBM_OK) {
-        HIPCHK(hipStreamSynchronize(d->stream));
-        float ms = 0;
-        double bulk = 0, near = 0;
-        for (uint64_t t = 0; t < nsteps; ++t) {
-            HIPCHK(hipEventElapsedTime(&ms, ev[4 * t + 0], ev[4 * t + 1]));
-            bulk += ms;
-            HIPCHK(hipEventElapsedTime(&ms, ev[4 * t + 2], ev[4 * t + 3]));
-            near += ms;
-        }
-        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[4 * nsteps - 1]));
-        out[0] = ms;
-        out[1] = bulk;
-        out[2] = near;
-        out[3] = static_cast<double>(nsteps);
+    Events ev(4 * nsteps);
+    HIPCHK(ev.create());
+    if (int rc = run_steps(d, nsteps, ev.data())) return rc;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    float ms = 0;
+    double bulk = 0, near = 0;
+    for (uint64_t t = 0; t < nsteps; ++t) {
+        HIPCHK(hipEventElapsedTime(&ms, ev[4 * t + 0], ev[4 * t + 1]));
+        bulk += ms;
+        HIPCHK(hipEventElapsedTime(&ms, ev[4 * t + 2], ev[4 * t + 3]));
+        near += ms;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    return rc;
+    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[4 * nsteps - 1]));
+    out[0] = ms;
+    out[1] = bulk;
+    out[2] = near;
+    out[3] = static_cast<double>(nsteps);
+    return LBM_OK;
 }
 
 int lbm_get_pdf(lbm_domain* d, double* out) {

@@ -160,19 +160,18 @@ int lbm_adj_forward(lbm_tape* t, const double* alpha, double* J) {
     if (int rc = check_handle(d)) return rc;
     if (alpha)
         if (int rc = upload_alpha(d, alpha)) return rc;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    Events ev(2);
+    HIPCHK(ev.create());
     HIPCHK(hipMemcpyAsync(d->buf[0], t->ckpt[0], d->field_bytes, hipMemcpyDeviceToDevice, d->stream));
     d->cur = 0;
-    HIPCHK(hipEventRecord(e0, d->stream));
+    HIPCHK(hipEventRecord(ev[0], d->stream));
     for (uint64_t s = 1; s <= t->T; ++s) {
         if (int rc = forward_step(d, d->buf[d->cur], d->buf[d->cur ^ 1], s == 1 && t->start_mat)) return rc;
         d->cur ^= 1;
         if (s % t->k == 0 && s / t->k < t->nck)
             HIPCHK(hipMemcpyAsync(t->ckpt[s / t->k], d->buf[d->cur], d->field_bytes, hipMemcpyDeviceToDevice, d->stream));
     }
-    HIPCHK(hipEventRecord(e1, d->stream));
+    HIPCHK(hipEventRecord(ev[1], d->stream));
     d->materialised = false;
     HIPCHK(t->ops->objective(d->L, d->buf[d->cur], d->d_kind, t->objective, t->plane_x, t->d_target,
                              d->d_partial, d->nred, d->d_out4, d->stream));
@@ -180,9 +179,7 @@ int lbm_adj_forward(lbm_tape* t, const double* alpha, double* J) {
     HIPCHK(hipMemcpyAsync(out, d->d_out4, sizeof(out), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
     t->timing[2] = ms;
     t->timing[3] = static_cast<double>(t->T);
     if (J) *J = out[0];
@@ -218,8 +215,8 @@ int lbm_adj_backward(lbm_tape* t, double dJ, double* grad_alpha, double grad_wal
     a.plane_x = t->plane_x;
     a.target = t->d_target;
 
-    std::vector<hipEvent_t> ev(3 * t->nck, nullptr);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    Events ev(3 * t->nck);
+    HIPCHK(ev.create());
     for (uint64_t si = t->nck; si-- > 0;) {
         const uint64_t start = si * t->k;
         const uint64_t end = std::min(start + t->k, t->T);
@@ -247,7 +244,6 @@ int lbm_adj_backward(lbm_tape* t, double dJ, double* grad_alpha, double grad_wal
         HIPCHK(hipEventElapsedTime(&ms, ev[3 * si + 1], ev[3 * si + 2]));
         adj += ms;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     t->timing[0] = rec;
     t->timing[1] = adj;
 

@@ -297,23 +297,20 @@ int lbm_batch_sync(lbm_batch* b) {
 int lbm_batch_time_steps(lbm_batch* b, uint64_t nsteps, double out[2]) {
     if (!b || !out || nsteps == 0) return fail(LBM_ERR_ARG, "bad arguments");
     HIPCHK(hipSetDevice(b->device));
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    Events ev(2);
+    HIPCHK(ev.create());
     // build (capture + instantiate) outside the timed region, then time one launch
     int parity, mat;
     if (int rc = check_parity(b, parity, mat)) return rc;
     if (int rc = sync_domains(b)) return rc;
     if (int rc = ensure_graph(b, nsteps, parity, mat)) return rc;
-    HIPCHK(hipEventRecord(e0, b->stream));
+    HIPCHK(hipEventRecord(ev[0], b->stream));
     HIPCHK(hipGraphLaunch(b->exec, b->stream));
-    HIPCHK(hipEventRecord(e1, b->stream));
+    HIPCHK(hipEventRecord(ev[1], b->stream));
     advance_domains(b, nsteps);
-    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventSynchronize(ev[1]));
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
     out[0] = ms;
     out[1] = static_cast<double>(b->ncells);
     return LBM_OK;
@@ -326,75 +323,95 @@ int lbm_batch_tune(lbm_batch* b, int trial_steps, double* best_ms) {
     if (int rc = check_parity(b, parity, mat)) return rc;
     if (int rc = sync_domains(b)) return rc;
     double best = 1e300;
-    int bv = b->V, bp = b->bx_policy, bb = b->bpc, bk = b->kernel;
+    const int v0 = b->V, p0 = b->bx_policy, b0 = b->bpc, k0 = b->kernel;
+    int bv = v0, bp = p0, bb = b0, bk = k0;
     // The trials step the real fields; save every domain's current field and
-    // restore it afterwards, so tuning leaves the state as it found it.
-    // (Trial steps write only fluid cells, which the next step overwrites in
-    // the other buffer.)
-    std::vector<void*> saved(b->doms.size(), nullptr);
-    std::vector<int> saved_cur(b->doms.size());
-    std::vector<bool> saved_mat(b->doms.size());
-    auto release = [&] {
-        for (void* p : saved) (void)hipFree(p);
-    };
-    for (size_t i = 0; i < b->doms.size(); ++i) {
-        lbm_domain* d = b->doms[i];
-        if (hipMalloc(&saved[i], d->field_bytes) != hipSuccess) {
-            release();
-            return fail(LBM_ERR_ALLOC, "tuning needs a copy of every field");
-        }
-        HIPCHK(hipMemcpyAsync(saved[i], d->buf[d->cur], d->field_bytes, hipMemcpyDeviceToDevice, b->stream));
-        saved_cur[i] = d->cur;
-        saved_mat[i] = d->materialised;
+    // restore it afterwards, on every exit, so tuning leaves the state as it
+    // found it. (Trial steps write only fluid cells, which the next step
+    // overwrites in the other buffer.)
+    const size_t nd = b->doms.size();
+    std::vector<void*> saved(nd, nullptr);
+    std::vector<int> saved_cur(nd);
+    std::vector<bool> saved_mat(nd);
+    for (size_t i = 0; i < nd; ++i) {
+        saved_cur[i] = b->doms[i]->cur;
+        saved_mat[i] = b->doms[i]->materialised;
     }
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
+    Events ev(2);
     // Kernels are compiled without FMA contraction, so a cell's arithmetic is
     // the same for every V, tile shape and grid size: the search changes
     // speed only, never results.
-    for (int kern : {BK_CELLS, BK_ROWS_FUSED, BK_ROWS_SEPARATE}) {
-        for (int i = 0; i < 4 && b->ops->vecs[i]; ++i)
-            for (int pol = 0; pol < 2; ++pol)
-                for (int bpc : {0, 4, 8, 16}) {
-                    // cell tiles have one shape: try it once per grid size
-                    if (kern == BK_CELLS && (i > 0 || pol > 0)) continue;
-                    if (kern != BK_CELLS && b->ops->vecs[i] > 1 && b->any_compact) continue;
-                    if (int rc = configure(b, b->ops->vecs[i], pol, bpc, kern)) return rc;
-                    // trial runs advance the fields; tuning happens before the timed run
-                    int p, m;
-                    check_parity(b, p, m);
-                    HIPCHK(enqueue(b, 1, p, m));  // warm-up
-                    advance_domains(b, 1);
-                    check_parity(b, p, m);
-                    HIPCHK(hipEventRecord(e0, b->stream));
-                    HIPCHK(enqueue(b, static_cast<uint64_t>(trial_steps), p, m));
-                    HIPCHK(hipEventRecord(e1, b->stream));
-                    HIPCHK(hipEventSynchronize(e1));
-                    advance_domains(b, static_cast<uint64_t>(trial_steps));
-                    float ms = 0;
-                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                    const double per = ms / trial_steps;
-                    if (per < best) {
-                        best = per;
-                        bv = b->V;
-                        bp = pol;
-                        bb = bpc;
-                        bk = kern;
+    auto search = [&]() -> int {
+        for (size_t i = 0; i < nd; ++i) {
+            lbm_domain* d = b->doms[i];
+            if (hipMalloc(&saved[i], d->field_bytes) != hipSuccess) {
+                saved[i] = nullptr;
+                return fail(LBM_ERR_ALLOC, "tuning needs a copy of every field");
+            }
+            HIPCHK(hipMemcpyAsync(saved[i], d->buf[d->cur], d->field_bytes, hipMemcpyDeviceToDevice, b->stream));
+        }
+        HIPCHK(ev.create());
+        for (int kern : {BK_CELLS, BK_ROWS_FUSED, BK_ROWS_SEPARATE}) {
+            for (int i = 0; i < 4 && b->ops->vecs[i]; ++i)
+                for (int pol = 0; pol < 2; ++pol)
+                    for (int bpc : {0, 4, 8, 16}) {
+                        // cell tiles have one shape: try it once per grid size
+                        if (kern == BK_CELLS && (i > 0 || pol > 0)) continue;
+                        if (kern != BK_CELLS && b->ops->vecs[i] > 1 && b->any_compact) continue;
+                        if (int rc = configure(b, b->ops->vecs[i], pol, bpc, kern)) return rc;
+                        // trial runs advance the fields; the saved copies restore them
+                        int p, m;
+                        check_parity(b, p, m);
+                        HIPCHK(enqueue(b, 1, p, m));  // warm-up
+                        advance_domains(b, 1);
+                        check_parity(b, p, m);
+                        HIPCHK(hipEventRecord(ev[0], b->stream));
+                        HIPCHK(enqueue(b, static_cast<uint64_t>(trial_steps), p, m));
+                        HIPCHK(hipEventRecord(ev[1], b->stream));
+                        HIPCHK(hipEventSynchronize(ev[1]));
+                        advance_domains(b, static_cast<uint64_t>(trial_steps));
+                        float ms = 0;
+                        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+                        const double per = ms / trial_steps;
+                        if (per < best) {
+                            best = per;
+                            bv = b->V;
+                            bp = pol;
+                            bb = bpc;
+                            bk = kern;
+                        }
                     }
-                }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    for (size_t i = 0; i < b->doms.size(); ++i) {
+        }
+        return LBM_OK;
+    };
+    int rc = search();
+    const std::string err = rc ? last_error() : std::string();
+    // Restore every domain (the fields only if every copy was taken) and free
+    // what the search took, whether it finished or not.
+    bool all_saved = true;
+    for (void* p : saved) all_saved &= (p != nullptr);
+    for (size_t i = 0; i < nd; ++i) {
         lbm_domain* d = b->doms[i];
         d->cur = saved_cur[i];
         d->materialised = saved_mat[i];
-        HIPCHK(hipMemcpyAsync(d->buf[d->cur], saved[i], d->field_bytes, hipMemcpyDeviceToDevice, b->stream));
+        if (all_saved) {
+            const hipError_t e = hipMemcpyAsync(d->buf[d->cur], saved[i], d->field_bytes, hipMemcpyDeviceToDevice,
+                                                b->stream);
+            if (e != hipSuccess && !rc) rc = fail(LBM_ERR_HIP, std::string("restoring a field: ") + hipGetErrorString(e));
+        }
     }
-    HIPCHK(hipStreamSynchronize(b->stream));
-    release();
-    if (int rc = configure(b, bv, bp, bb, bk)) return rc;
+    {
+        const hipError_t e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess && !rc) rc = fail(LBM_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+    }
+    for (void* p : saved)
+        if (p) (void)hipFree(p);
+    if (rc) {
+        // keep the configuration the batch had, and the first error's message
+        (void)configure(b, v0, p0, b0, k0);
+        return fail(rc, err.empty() ? last_error() : err);
+    }
+    if (int c = configure(b, bv, bp, bb, bk)) return c;
     if (best_ms) *best_ms = best;
     return LBM_OK;
 }
@@ -580,10 +597,9 @@ int lbm_batch_adj_forward(lbm_batch_tape* t, double* J) {
         d->cur = 0;
         d->materialised = t->start_mat;
     }
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, b->stream));
+    Events ev(2);
+    HIPCHK(ev.create());
+    HIPCHK(hipEventRecord(ev[0], b->stream));
     for (uint64_t si = 0; si < t->nck; ++si) {
         const uint64_t steps = std::min(t->k, t->T - si * t->k);
         int parity, mat;
@@ -598,7 +614,7 @@ int lbm_batch_adj_forward(lbm_batch_tape* t, double* J) {
                 HIPCHK(hipMemcpyAsync(dst, d->buf[d->cur], d->field_bytes, hipMemcpyDeviceToDevice, b->stream));
             }
     }
-    HIPCHK(hipEventRecord(e1, b->stream));
+    HIPCHK(hipEventRecord(ev[1], b->stream));
     for (size_t di = 0; di < b->doms.size(); ++di) t->views[di].gfinal = b->doms[di]->buf[b->doms[di]->cur];
     HIPCHK(hipMemcpyAsync(t->d_views, t->views.data(), sizeof(BatchAdjView) * t->views.size(), hipMemcpyHostToDevice,
                           b->stream));
@@ -609,9 +625,7 @@ int lbm_batch_adj_forward(lbm_batch_tape* t, double* J) {
     }
     HIPCHK(hipStreamSynchronize(b->stream));
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
     t->timing[2] = ms;
     t->timing[3] = static_cast<double>(t->T);
     if (J)
@@ -637,8 +651,8 @@ int lbm_batch_adj_backward(lbm_batch_tape* t, double* grad_wall) {
     a.tiles = t->d_tiles;
     a.ntiles = t->ntiles;
     const int grid = grid_for(t);
-    std::vector<hipEvent_t> ev(3 * t->nck, nullptr);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    Events ev(3 * t->nck);
+    HIPCHK(ev.create());
     for (uint64_t si = t->nck; si-- > 0;) {
         const uint64_t start = si * t->k;
         const uint64_t end = std::min(start + t->k, t->T);
@@ -671,7 +685,6 @@ int lbm_batch_adj_backward(lbm_batch_tape* t, double* grad_wall) {
         HIPCHK(hipEventElapsedTime(&ms, ev[3 * si + 1], ev[3 * si + 2]));
         adj += ms;
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     t->timing[0] = rec;
     t->timing[1] = adj;
     if (grad_wall)

@@ -32,6 +32,44 @@ inline int fail(int code, const std::string& msg) {
             return fail(LBM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// HIP events owned by a scope: destroyed when it ends, error returns included.
+class Events {
+public:
+    explicit Events(size_t n) : ev_(n, nullptr) {}
+    ~Events() {
+        for (hipEvent_t e : ev_)
+            if (e) (void)hipEventDestroy(e);
+    }
+    Events(const Events&) = delete;
+    Events& operator=(const Events&) = delete;
+    hipError_t create() {
+        for (hipEvent_t& e : ev_) {
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) {
+                e = nullptr;
+                return r;
+            }
+        }
+        return hipSuccess;
+    }
+    hipEvent_t operator[](size_t i) const { return ev_[i]; }
+    hipEvent_t* data() { return ev_.data(); }
+
+private:
+    std::vector<hipEvent_t> ev_;
+};
+
+// Device memory owned by a scope: freed when it ends, error returns included.
+struct DeviceTemp {
+    void* p = nullptr;
+    DeviceTemp() = default;
+    ~DeviceTemp() {
+        if (p) (void)hipFree(p);
+    }
+    DeviceTemp(const DeviceTemp&) = delete;
+    DeviceTemp& operator=(const DeviceTemp&) = delete;
+};
+
 struct HostCond {
     int kind;
     double vel[3];
