@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define LBM_HIP_ABI_VERSION 3
+#define LBM_HIP_ABI_VERSION 4
 
 /* Status codes. */
 #define LBM_OK 0
@@ -307,6 +307,8 @@ typedef struct lbm_info {
     int nconds;                /* distinct boundary conditions */
     int cell_tiles;            /* 1: steps run as 64-cell tiles per wavefront (k_cells) */
     int layout;                /* enum lbm_layout */
+    int steps_per_pass;        /* 2: lbm_step runs pairs of steps as one pass (k_bulk2) */
+    int z_chunk;               /* planes per block of the two-step pass */
 } lbm_info;
 int lbm_get_info(lbm_domain* d, lbm_info* info);
 
@@ -317,6 +319,16 @@ int lbm_get_info(lbm_domain* d, lbm_info* info);
  * the near-boundary cells. block_x: threads along x per row-kernel block (a
  * divisor of 256; 0 = automatic). Results are the same bits either way. */
 int lbm_set_tuning(lbm_domain* d, int vec, int block_x);
+
+/* Time steps per pass over the field. Where every interior cell is bulk
+ * (periodic along all three axes, no boundary cells) and the storage mode is
+ * fp32, fp16 or bf16, lbm_step can run two steps in one pass: the field is
+ * read once and written once for both, with the intermediate state held on
+ * chip. Results are the same bits as two single steps. steps_per_pass: 1 or 2
+ * (default 2; a domain where the pass does not apply steps singly).
+ * z_chunk: planes per block of the pass (0 = automatic). No reference
+ * counterpart: a scheduling choice of this implementation. */
+int lbm_set_time_blocking(lbm_domain* d, int steps_per_pass, int z_chunk);
 
 #ifdef __cplusplus
 }

@@ -46,7 +46,7 @@ class Info(ctypes.Structure):
                 ("ncells_near", c_uint64), ("bytes_per_value", c_uint64),
                 ("field_bytes", c_uint64), ("device_bytes", c_uint64), ("vec", c_int),
                 ("block_x", c_int), ("block_y", c_int), ("nconds", c_int), ("cell_tiles", c_int),
-                ("layout", c_int)]
+                ("layout", c_int), ("steps_per_pass", c_int), ("z_chunk", c_int)]
 
 
 # (name, restype, argtypes) of every exported entry point in include/lbm_hip.h
@@ -93,6 +93,7 @@ SIGNATURES = [
     ("lbm_batch_adj_timing", c_int, [c_void_p, POINTER(c_double)]),
     ("lbm_get_info", c_int, [c_void_p, POINTER(Info)]),
     ("lbm_set_tuning", c_int, [c_void_p, c_int, c_int]),
+    ("lbm_set_time_blocking", c_int, [c_void_p, c_int, c_int]),
 ]
 
 _lib = None
@@ -265,6 +266,11 @@ class Domain:
 
     def set_tuning(self, vec: int = 0, block_x: int = 0) -> None:
         _check(load().lbm_set_tuning(self._h, vec, block_x))
+
+    def set_time_blocking(self, steps_per_pass: int = 2, z_chunk: int = 0) -> None:
+        """Two steps per pass for bulk-only (fully periodic) domains in
+        fp32/fp16/bf16 storage (default), or 1 for single steps."""
+        _check(load().lbm_set_time_blocking(self._h, steps_per_pass, z_chunk))
 
 
 class AdjDesc(ctypes.Structure):

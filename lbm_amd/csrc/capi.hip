@@ -125,6 +125,28 @@ int make_plan(lbm_domain* d) {
               p.nblocks_cells <= 0x7fffffffLL && (plane_cells + 63) / 64 <= 0x7fffffffLL;
     if (!p.cells && d->compact && V > 1)
         return fail(LBM_ERR_UNSUPPORTED, "packed rows need the cell-tile kernel or 1 cell per thread");
+    // Two steps per pass where every interior cell is bulk: no cell pulls from
+    // a non-fluid source, so all sources are interior cells (periodic images).
+    p.tb_ntx = (d->L.xl + TB_TX - 1) / TB_TX;
+    p.tb_nty = (d->L.yl + TB_TY - 1) / TB_TY;
+    const int64_t tiles = static_cast<int64_t>(p.tb_ntx) * p.tb_nty;
+    // By default the z chunks make the grid one round of resident blocks: a
+    // block marches through its planes, so fewer, longer blocks recompute
+    // fewer chunk-boundary planes, and one round leaves no tail.
+    int zc = d->tune_zc;
+    if (!zc && o->step2) {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d->desc.device) != hipSuccess)
+            ncu = 0;
+        const int64_t resident = std::max<int64_t>(1, static_cast<int64_t>(ncu) * o->step2_blocks_per_cu(p.brink));
+        const int64_t chunks = std::max<int64_t>(1, resident / tiles);
+        zc = static_cast<int>((d->L.zl + chunks - 1) / chunks);
+    }
+    p.tb_zc = std::max(1, std::min(zc, d->L.zl));
+    p.tb_nblocks = tiles * ((d->L.zl + p.tb_zc - 1) / p.tb_zc);
+    // (element offsets within a plane are 32-bit in the pass)
+    p.pairs = d->tune_pass == 2 && o->step2 != nullptr && !d->aa && d->nb == 0 && !p.flags && p.has_bulk &&
+              p.tb_nblocks <= 0x7fffffffLL && d->L.plane < (int64_t{1} << 32);
     return LBM_OK;
 }
 
@@ -519,7 +541,20 @@ static int run_steps(lbm_domain* d, uint64_t nsteps, hipEvent_t* ev) {
         }
         return LBM_OK;
     }
-    for (uint64_t t = 0; t < nsteps; ++t) {
+    uint64_t t = 0;
+    if (d->plan.pairs)
+        for (; t + 2 <= nsteps; t += 2) {
+            // (every source is an interior cell: the materialised flag has no effect)
+            const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur ^ 1], 0);
+            hipEvent_t* e4 = ev ? ev + 4 * t : nullptr;
+            hipError_t e = d->ops->step2(a, d->plan, d->stream, e4);
+            if (e4)  // the pass's second step: no launch of its own
+                for (int k = 4; k < 8; ++k) (void)hipEventRecord(e4[k], d->stream);
+            if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("two-step launch: ") + hipGetErrorString(e));
+            d->cur ^= 1;
+            d->materialised = false;
+        }
+    for (; t < nsteps; ++t) {
         const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur ^ 1], d->materialised ? 1 : 0);
         hipError_t e = d->ops->step(a, d->plan, d->stream, ev ? ev + 4 * t : nullptr);
         if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e));
@@ -765,6 +800,18 @@ int lbm_get_info(lbm_domain* d, lbm_info* info) {
     info->nconds = d->nconds_dev;
     info->cell_tiles = (d->plan.cells || d->aa) ? 1 : 0;
     info->layout = d->aa ? LBM_LAYOUT_AA : LBM_LAYOUT_AB;
+    info->steps_per_pass = d->plan.pairs ? 2 : 1;
+    info->z_chunk = d->plan.tb_zc;
+    return LBM_OK;
+}
+
+int lbm_set_time_blocking(lbm_domain* d, int steps_per_pass, int z_chunk) {
+    if (!d) return fail(LBM_ERR_ARG, "null domain handle");
+    if (steps_per_pass != 1 && steps_per_pass != 2) return fail(LBM_ERR_ARG, "steps_per_pass must be 1 or 2");
+    if (z_chunk < 0) return fail(LBM_ERR_ARG, "z_chunk must be >= 0");
+    d->tune_pass = steps_per_pass;
+    d->tune_zc = z_chunk;
+    if (d->finalized) return make_plan(d);
     return LBM_OK;
 }
 

@@ -150,6 +150,7 @@ struct lbm_domain {
     hipStream_t stream = nullptr;
     StepPlan plan{};
     int tune_vec = 0, tune_bx = 0;
+    int tune_pass = 2, tune_zc = 0;  // steps per pass, planes per block of the pass
     double* d_partial = nullptr;
     double* d_out4 = nullptr;
     int nred = 0;

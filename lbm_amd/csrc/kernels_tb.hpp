@@ -1,0 +1,198 @@
+// Two time steps per pass (temporal blocking) for domains whose every
+// interior cell is BULK: periodic along all three axes, no boundary cells.
+//
+// A pass reads the field g0 once and writes g2 once, where two single steps
+// read and write it twice each. Per cell the arithmetic is that of k_bulk
+// and the intermediate g1 is rounded to the storage type, so a pass equals
+// two single steps bit for bit.
+//
+// A block of 256 threads owns a TX x TY tile of cells in x-y and a chunk of
+// z planes [z0, z1]. It marches through the planes zs = z0 - 1 .. z1 + 1:
+//   (1) step 1 on plane zs for the tile and the ring of cells around it in
+//       x-y (the (TX + 2) x (TY + 2) patch), pulling g0 from global memory;
+//       every patch cell sends g1_q to the tile cell x + c_q of plane
+//       zs + cz_q (the cell that pulls it in step 2) through LDS;
+//   (2) step 2 on plane zs - 1, whose Q values have all arrived: collide and
+//       store g2.
+// A value sent with cz = -1 is read in the same iteration, cz = 0 in the next
+// and cz = +1 in the one after. The cz >= 0 values are held in registers
+// across (2) and written afterwards, so LDS holds one plane of slots for the
+// cz = -1 and cz = 0 directions and two planes for cz = +1.
+//
+// Coordinates are kept unwrapped (x0 - 1 .. x0 + TX, zs = z0 - 1 .. z1 + 1)
+// for the LDS bookkeeping and wrapped onto the periodic box only to address
+// global memory, which keeps boxes of one or two cells along an axis exact.
+#pragma once
+
+#include "kernels.hpp"
+#include "ops.hpp"
+
+namespace lbm_amd {
+
+constexpr int TB_PX = TB_TX + 2;
+constexpr int TB_PY = TB_TY + 2;
+constexpr int TB_NP = TB_PX * TB_PY;            // patch cells of one plane
+constexpr int TB_PASS = (TB_NP + 255) / 256;    // step-1 cells per thread
+constexpr int TB_CELLS = TB_TX * TB_TY;
+constexpr int TB_OUT = TB_CELLS / 256;          // step-2 cells per thread
+static_assert(TB_CELLS % 256 == 0, "step 2 gives every thread the same number of cells");
+
+// LDS slot of direction q for the destination plane zd.
+template <class L>
+struct TbSlots {
+    static constexpr int count(int cz) {
+        int n = 0;
+        for (int q = 0; q < L::Q; ++q) n += (L::cz(q) == cz) ? 1 : 0;
+        return n;
+    }
+    static constexpr int NM = count(-1), N0 = count(0), NP = count(1);
+    static constexpr int NSLOT = NM + N0 + 2 * NP;
+    static constexpr int rank(int q) {
+        int r = 0;
+        for (int k = 0; k < q; ++k) r += (L::cz(k) == L::cz(q)) ? 1 : 0;
+        return r;
+    }
+    static constexpr int base(int q) {
+        return L::cz(q) < 0 ? rank(q) : (L::cz(q) == 0 ? NM + rank(q) : NM + N0 + 2 * rank(q));
+    }
+    template <int q>
+    static __device__ __forceinline__ int slot(int zd) {
+        if constexpr (L::cz(q) > 0) return base(q) + (zd & 1);
+        else return base(q);
+    }
+};
+
+template <class L, Mode M>
+constexpr size_t tb_lds_bytes() {
+    return static_cast<size_t>(TbSlots<L>::NSLOT) * TB_CELLS * sizeof(typename Io<M, L>::T);
+}
+
+template <class L, Mode M, bool BRINK>
+__device__ __forceinline__ void tb_collide(const StepArgs& a, typename Io<M, L>::Calc (&f)[L::Q],
+                                           typename Io<M, L>::Calc alpha) {
+    using Calc = typename Io<M, L>::Calc;
+    collide_shifted<L, Calc, BRINK, kRcp<M>>(f, static_cast<Calc>(a.omega), BRINK ? alpha : Calc(0),
+                                             static_cast<Calc>(a.tau * a.g[0]), static_cast<Calc>(a.tau * a.g[1]),
+                                             static_cast<Calc>(a.tau * a.g[2]));
+}
+
+// Grid: blockIdx.x = tx + ntx * (ty + nty * chunk); chunk c covers planes
+// 1 + c * zc .. min((c + 1) * zc, zl).
+template <class L, Mode M, bool BRINK>
+__global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int zc) {
+    using IO = Io<M, L>;
+    using T = typename IO::T;
+    using Calc = typename IO::Calc;
+    using S = TbSlots<L>;
+    constexpr int Q = L::Q;
+    __shared__ T lds[S::NSLOT][TB_CELLS];
+
+    const FieldLayout& Ly = a.L;
+    const int tid = threadIdx.x;
+    int b = blockIdx.x;
+    const int x0 = 1 + (b % ntx) * TB_TX;
+    b /= ntx;
+    const int y0 = 1 + (b % nty) * TB_TY;
+    b /= nty;
+    const int z0 = 1 + b * zc;
+    const int z1 = min(z0 + zc - 1, Ly.zl);
+    const T* __restrict__ src = static_cast<const T*>(a.src);
+    T* __restrict__ dst = static_cast<T*>(a.dst);
+    const Calc* __restrict__ alpha = static_cast<const Calc*>(a.alpha);
+    // every element offset of a plane fits 32 bits (the host checks)
+    const uint32_t py = static_cast<uint32_t>(Ly.px), pz = static_cast<uint32_t>(Ly.pxy);
+
+    // this thread's patch cells: tile-local (pi, pj) in [-1, TX] x [-1, TY]
+    int pi[TB_PASS], pj[TB_PASS], gx[TB_PASS], gy[TB_PASS];
+    bool live[TB_PASS];
+#pragma unroll
+    for (int k = 0; k < TB_PASS; ++k) {
+        const int p = k * 256 + tid;
+        pi[k] = p % TB_PX - 1;
+        pj[k] = p / TB_PX - 1;
+        const int ux = x0 + pi[k], uy = y0 + pj[k];
+        live[k] = p < TB_NP && ux <= Ly.xl + 1 && uy <= Ly.yl + 1;
+        gx[k] = wrap1(ux, Ly.xl);
+        gy[k] = wrap1(uy, Ly.yl);
+    }
+
+    T hold[TB_PASS][Q];
+    for (int zs = z0 - 1; zs <= z1 + 1; ++zs) {
+        // (1) step 1 on plane zs
+        const int gz = wrap1(zs, Ly.zl);
+        const uint32_t zs3[3] = {static_cast<uint32_t>(Ly.off) + pz * wrap1(gz - 1, Ly.zl),
+                                 static_cast<uint32_t>(Ly.off) + pz * gz,
+                                 static_cast<uint32_t>(Ly.off) + pz * wrap1(gz + 1, Ly.zl)};
+#pragma unroll
+        for (int k = 0; k < TB_PASS; ++k) {
+            if (!live[k]) continue;
+            // 32-bit element offsets within a plane (the plane base is uniform)
+            const uint32_t xs[3] = {static_cast<uint32_t>(wrap1(gx[k] - 1, Ly.xl)), static_cast<uint32_t>(gx[k]),
+                                    static_cast<uint32_t>(wrap1(gx[k] + 1, Ly.xl))};
+            const uint32_t ys[3] = {py * wrap1(gy[k] - 1, Ly.yl), py * gy[k], py * wrap1(gy[k] + 1, Ly.yl)};
+            Calc f[Q];
+            static_for<0, Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                // source x - c_q: index 0 for c = +1, 1 for 0, 2 for -1
+                constexpr int ix = 1 - L::cx(q), iy = 1 - L::cy(q), iz = 1 - L::cz(q);
+                const uint32_t o = zs3[iz] + ys[iy] + xs[ix];
+                f[q] = IO::template to_calc<q>(src[q * Ly.plane + o]);
+            });
+            Calc al = Calc(0);
+            if constexpr (BRINK) al = alpha[zs3[1] + ys[1] + xs[1]];
+            tb_collide<L, M, BRINK>(a, f, al);
+            static_for<0, Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const T v = IO::template from_calc<q>(f[q]);
+                if constexpr (L::cz(q) < 0) {
+                    const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
+                    if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY)
+                        lds[S::template slot<q>(zs - 1)][dj * TB_TX + di] = v;
+                } else {
+                    hold[k][q] = v;
+                }
+            });
+        }
+        __syncthreads();
+        // (2) step 2 on plane zs - 1
+        const int zo = zs - 1;
+        if (zo >= z0) {
+#pragma unroll
+            for (int k = 0; k < TB_OUT; ++k) {
+                const int c = k * 256 + tid;
+                const int ox = x0 + c % TB_TX, oy = y0 + c / TB_TX;
+                if (ox <= Ly.xl && oy <= Ly.yl) {
+                    Calc f[Q];
+                    static_for<0, Q>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        f[q] = IO::template to_calc<q>(lds[S::template slot<q>(zo)][c]);
+                    });
+                    const uint32_t e = static_cast<uint32_t>(Ly.off) + pz * zo + py * oy + ox;
+                    Calc al = Calc(0);
+                    if constexpr (BRINK) al = alpha[e];
+                    tb_collide<L, M, BRINK>(a, f, al);
+                    static_for<0, Q>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        store_stream1<T>(dst + q * Ly.plane + e, IO::template from_calc<q>(f[q]));
+                    });
+                }
+            }
+        }
+        __syncthreads();
+        // sends with cz >= 0, read by step 2 of the next two iterations
+#pragma unroll
+        for (int k = 0; k < TB_PASS; ++k) {
+            if (!live[k]) continue;
+            static_for<0, Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (L::cz(q) >= 0) {
+                    const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
+                    if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY)
+                        lds[S::template slot<q>(zs + L::cz(q))][dj * TB_TX + di] = hold[k][q];
+                }
+            });
+        }
+    }
+}
+
+}  // namespace lbm_amd

@@ -14,6 +14,10 @@ namespace lbm_amd {
 
 struct GhostDims;
 
+// Tile of the two-step pass (k_bulk2): cells along x and y of one block.
+constexpr int TB_TX = 64;
+constexpr int TB_TY = 8;
+
 struct StepPlan {
     int V;          // cells per thread along x in k_bulk
     int BX, BY;     // k_bulk block shape (BX * BY == 256)
@@ -28,6 +32,11 @@ struct StepPlan {
     int tpp;               // 64-cell tiles per z plane
     int64_t nblocks_cells; // blocks of 4 wavefronts
     double rxl;            // 1 / xl
+    // Two steps per pass (k_bulk2): 64 x 8 cell tiles in x-y, tb_zc planes
+    // per block along z.
+    bool pairs;
+    int tb_ntx, tb_nty, tb_zc;
+    int64_t tb_nblocks;
 };
 
 struct ModeOps {
@@ -40,6 +49,11 @@ struct ModeOps {
     // bulk + near for one step (src/dst in args). ev (may be null) receives
     // records before/after the bulk and near launches: ev[0..3].
     hipError_t (*step)(const StepArgs& a, const StepPlan& p, hipStream_t s, hipEvent_t* ev);
+    // two steps in one pass for bulk-only domains (null where not built);
+    // ev as for step
+    hipError_t (*step2)(const StepArgs& a, const StepPlan& p, hipStream_t s, hipEvent_t* ev);
+    // blocks of the two-step pass resident per CU (0 where not built)
+    int (*step2_blocks_per_cu)(bool brink);
     hipError_t (*get_pdf)(const FieldLayout& Ly, const void* src, double* out, int nx, int ny,
                           int nz, hipStream_t s);
     hipError_t (*set_pdf)(const FieldLayout& Ly, void* dst, const double* in, int nx, int ny,

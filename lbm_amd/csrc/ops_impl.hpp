@@ -5,6 +5,7 @@
 #include "kernels.hpp"
 #include "kernels_aa.hpp"
 #include "kernels_fields.hpp"
+#include "kernels_tb.hpp"
 #include "ops.hpp"
 
 namespace lbm_amd {
@@ -62,6 +63,41 @@ struct OpsImpl {
         }
         if (ev) (void)hipEventRecord(ev[3], s);
         return hipGetLastError();
+    }
+
+    // the two-step pass: 4- and 2-byte storage whose LDS slots fit 64 KiB
+    static constexpr bool tb_built() {
+        return (M == Mode::F32 || M == Mode::F16 || M == Mode::BF16) && tb_lds_bytes<L, M>() <= 65536;
+    }
+    static hipError_t step2_impl(const StepArgs& a, const StepPlan& p, hipStream_t s, hipEvent_t* ev) {
+        if constexpr (tb_built()) {
+            if (ev) (void)hipEventRecord(ev[0], s);
+            const dim3 grid(static_cast<uint32_t>(p.tb_nblocks));
+            if (p.brink)
+                hipLaunchKernelGGL((k_bulk2<L, M, true>), grid, dim3(256), 0, s, a, p.tb_ntx, p.tb_nty, p.tb_zc);
+            else
+                hipLaunchKernelGGL((k_bulk2<L, M, false>), grid, dim3(256), 0, s, a, p.tb_ntx, p.tb_nty, p.tb_zc);
+            if (ev) {
+                (void)hipEventRecord(ev[1], s);
+                (void)hipEventRecord(ev[2], s);
+                (void)hipEventRecord(ev[3], s);
+            }
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
+
+    static int step2_occupancy(bool brink) {
+        int n = 0;
+        if constexpr (tb_built()) {
+            const hipError_t e = brink ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                             &n, reinterpret_cast<const void*>(&k_bulk2<L, M, true>), 256, 0)
+                                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                             &n, reinterpret_cast<const void*>(&k_bulk2<L, M, false>), 256, 0);
+            if (e != hipSuccess) n = 0;
+        }
+        return n;
     }
 
     static hipError_t get_pdf(const FieldLayout& Ly, const void* src, double* out, int nx, int ny,
@@ -135,6 +171,8 @@ struct OpsImpl {
         const int v[] = {V0, Vs...};
         for (int i = 0; i < 4; ++i) o.vecs[i] = i < static_cast<int>(sizeof(v) / sizeof(int)) ? v[i] : 0;
         o.step = &step_impl<V0, Vs...>;
+        o.step2 = tb_built() ? &step2_impl : nullptr;
+        o.step2_blocks_per_cu = &step2_occupancy;
         o.get_pdf = &get_pdf;
         o.set_pdf = &set_pdf;
         o.moments = &moments;

@@ -38,7 +38,7 @@ def test_binding_covers_every_declared_symbol():
 
 def test_host_only_calls():
     lib = capi.load()
-    assert lib.lbm_abi_version() == 3
+    assert lib.lbm_abi_version() == 4
     d = capi.Desc()
     assert lib.lbm_desc_init(ctypes.byref(d)) == 0
     assert d.q == 19 and d.tau == 1.0 and d.spacing[2] == 1.0
