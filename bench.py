@@ -3,13 +3,18 @@
 Workload: periodic 256^3 D3Q19 box, BGK tau = 0.8, 2-D Taylor-Green vortex
 (u0 = 0.02, k = 2 pi / 256) plus a seeded density perturbation 1e-4 N(0,1),
 shifted distributions stored as fp32 (the headline) and as fp16 / bf16, all
-with fp32 arithmetic. A step is one fused pull-stream + collide pass over all
-16.8 M cells; inputs are resident in HBM when timing starts.
+with fp32 arithmetic. A step is one pull-stream + collide update of all
+16.8 M cells; inputs are resident in HBM when timing starts. lbm_step runs
+this bulk-only box two steps per pass (k_bulk2: the field is read and
+written once per pair of steps, the intermediate state stays on chip); each
+mode is also timed in single steps (k_bulk).
 
 Prints one JSON line: MLUPS of the fp32 mode (interior cells x steps / s),
 the roofline of the dominant kernel (algorithmic bytes 2*Q*sizeof(storage)
-per cell over its HIP-event-timed average launch), the fp16/bf16 modes and a
-CPU baseline (the fp64 CPU oracle, OpenMP on the host cores, bounded sample).
+per cell and step, times the cell-steps of one launch, over its
+HIP-event-timed average launch; beside it the HBM bytes rocprofv3 measured
+per launch), the fp16/bf16 modes and a CPU baseline (the fp64 CPU oracle,
+OpenMP on the host cores, bounded sample).
 Further legs ride on the same line:
 - "inplace_aa": the same workload in the in-place (AA) layout, one buffer;
 - "adjoint": the inverse problem on a periodic 128^3 D3Q19 fp32 box with a
@@ -60,20 +65,22 @@ def tgv(n: int, seed: int = 12345):
     return rho, u
 
 
-def traffic_from_profiles(mode: str):
-    """HBM bytes per bulk-kernel launch measured by rocprofv3 --pmc for this
-    workload (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel
-    calibration), from the committed summary; None when absent."""
-    p = ROOT / "profiles" / f"pmc_bulk_d3q19_256_{mode}.txt"
+def traffic_from_profiles(mode: str, kernel: str = "bulk"):
+    """HBM bytes per launch of the kernel (bulk: k_bulk, pass: k_bulk2)
+    measured by rocprofv3 --pmc for this workload (FETCH_SIZE + WRITE_SIZE,
+    each scaled by the copy-kernel calibration), from the committed summary;
+    None when absent."""
+    p = ROOT / "profiles" / f"pmc_{kernel}_d3q19_256_{mode}.txt"
     if not p.exists():
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text(), flags=re.M)
     return float(m.group(1)) if m else None
 
 
-def run_mode(capi, mode: str, steps: int, warmup: int, rho, u):
+def run_mode(capi, mode: str, steps: int, warmup: int, rho, u, steps_per_pass: int = 2):
     d = capi.Domain(N, N, N, TAU, q=Q, storage=mode, periodic=(True, True, True))
     d.finalize()
+    d.set_time_blocking(steps_per_pass)
     d.init_equilibrium(rho, u)
     d.step(warmup)
     d.sync()
@@ -85,9 +92,15 @@ def run_mode(capi, mode: str, steps: int, warmup: int, rho, u):
     mass = d.conservation()
     info = d.info()
     d.close()
-    kernel_ms = ev["bulk_ms"] / steps
-    alg_bytes = 2 * Q * BYTES[mode] * cells
+    spp = info["steps_per_pass"]
+    # launches in the timed region (an odd count adds one single step)
+    launches = steps // spp + steps % spp
+    kernel_ms = ev["bulk_ms"] / launches
+    alg_bytes = 2 * Q * BYTES[mode] * cells * spp
     return {
+        "kernel": "k_bulk2" if spp == 2 else "k_bulk",
+        "steps_per_pass": spp,
+        "z_chunk": info["z_chunk"] if spp == 2 else None,
         "mlups": cells * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "event_ms_per_step": ev["total_ms"] / steps,
@@ -101,11 +114,13 @@ def run_mode(capi, mode: str, steps: int, warmup: int, rho, u):
 
 
 def aa_leg(capi, mode: str, steps: int, rho, u):
-    """The same workload in the in-place (AA) layout: one distribution buffer."""
+    """The same workload in the in-place (AA) layout: one distribution buffer,
+    one step per launch; beside it the two-buffer layout in single steps."""
     out = {}
     for layout in ("ab", "aa"):
         d = capi.Domain(N, N, N, TAU, q=Q, storage=mode, periodic=(True, True, True), layout=layout)
         d.finalize()
+        d.set_time_blocking(1)
         d.init_equilibrium(rho, u)
         d.step(10)
         d.sync()
@@ -115,7 +130,8 @@ def aa_leg(capi, mode: str, steps: int, rho, u):
     return {"storage": mode, "steps": steps,
             "mlups": round(N ** 3 / (out["aa"]["kernel_ms"] * 1e3), 1),
             "kernel_ms": out["aa"]["kernel_ms"], "device_gb": out["aa"]["device_gb"],
-            "two_buffer_kernel_ms": out["ab"]["kernel_ms"], "two_buffer_device_gb": out["ab"]["device_gb"]}
+            "two_buffer_single_step_kernel_ms": out["ab"]["kernel_ms"],
+            "two_buffer_device_gb": out["ab"]["device_gb"]}
 
 
 def cpu_baseline(rho, u, steps: int = 4):
@@ -249,9 +265,12 @@ def main() -> None:
     rho, u = tgv(N)
     modes = [m for m in a.modes.split(",") if m]
     res = {m: run_mode(capi, m, a.steps, a.warmup, rho, u) for m in modes}
+    single = {m: run_mode(capi, m, a.steps, a.warmup, rho, u, steps_per_pass=1) for m in modes}
     head = res.get("f32") or res[modes[0]]
     head_mode = "f32" if "f32" in res else modes[0]
-    traffic = traffic_from_profiles(head_mode)
+    traffic = traffic_from_profiles(head_mode, "pass" if head["steps_per_pass"] == 2 else "bulk")
+    s_head = single[head_mode]
+    s_traffic = traffic_from_profiles(head_mode, "bulk")
     line = {
         "metric": "MLUPS + %HBM roofline, D3Q19 256^3 fp32 vs fp16-store",
         "value": round(head["mlups"], 1),
@@ -267,24 +286,41 @@ def main() -> None:
         "data": "synthetic (seeded Taylor-Green vortex + density noise, seed 12345)",
         "config": {"workload": "D3Q19 periodic 256^3 TGV, BGK tau=0.8, shifted f-w stored fp32",
                    "lattice": "D3Q19", "shape": [N, N, N], "tau": TAU, "storage": head_mode,
-                   "vec": head["vec"], "block": head["block"]},
+                   "steps_per_pass": head["steps_per_pass"], "z_chunk": head["z_chunk"]},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "k_bulk", "kernel_ms": round(head["kernel_ms"], 5),
+                     "kernel": head["kernel"], "kernel_ms": round(head["kernel_ms"], 5),
+                     "steps_per_launch": head["steps_per_pass"],
                      "alg_bytes_per_launch": head["alg_bytes"],
+                     # HBM bytes actually moved per launch over its duration
+                     "hbm_gbs_measured": (round(traffic / (head["kernel_ms"] * 1e-3) / 1e9, 1)
+                                          if traffic else None),
                      "copy_gbs": round(copy["gbs"], 1),
-                     "frac_of_copy": round(head["achieved_gbs"] / copy["gbs"], 4),
                      "q_plane_copy_gbs": round(qcopy["gbs"], 1),
-                     "frac_of_q_plane_copy": round(head["achieved_gbs"] / qcopy["gbs"], 4)},
-        "modes": {m: {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
+                     "single_step": {"kernel": s_head["kernel"], "kernel_ms": round(s_head["kernel_ms"], 5),
+                                     "achieved": round(s_head["achieved_gbs"], 1),
+                                     "frac": round(s_head["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                                     "traffic": s_traffic,
+                                     "alg_bytes_per_launch": s_head["alg_bytes"],
+                                     "frac_of_copy": round(s_head["achieved_gbs"] / copy["gbs"], 4),
+                                     "frac_of_q_plane_copy": round(s_head["achieved_gbs"] / qcopy["gbs"], 4),
+                                     "mlups": round(s_head["mlups"], 1),
+                                     "vec": s_head["vec"], "block": s_head["block"]}},
+        "modes": {m: {"mlups": round(r["mlups"], 1), "kernel": r["kernel"], "kernel_ms": round(r["kernel_ms"], 5),
                       "achieved_gbs": round(r["achieved_gbs"], 1),
                       "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                      "bytes_per_cell": 2 * Q * BYTES[m]} for m, r in res.items()},
+                      "bytes_per_cell_step": 2 * Q * BYTES[m],
+                      "single_step": {"mlups": round(single[m]["mlups"], 1),
+                                      "kernel_ms": round(single[m]["kernel_ms"], 5),
+                                      "achieved_gbs": round(single[m]["achieved_gbs"], 1),
+                                      "roofline_frac": round(single[m]["achieved_gbs"] / HBM_PEAK_GBS, 4)}}
+                  for m, r in res.items()},
     }
     if "f16" in res and "f32" in res:
         line["f16_over_f32"] = round(res["f16"]["mlups"] / res["f32"]["mlups"], 3)
+        line["f16_over_f32_single_step"] = round(single["f16"]["mlups"] / single["f32"]["mlups"], 3)
     if not a.no_aa:
         line["inplace_aa"] = aa_leg(capi, head_mode, min(a.steps, 200), rho, u)
     line["adjoint"] = None if a.no_adjoint else adjoint_leg(capi)

@@ -158,9 +158,11 @@ int lbm_moments(lbm_domain* d, double* rho, double* u);
  * (x, y, z), reduced on the device. */
 int lbm_conservation(lbm_domain* d, double out[4]);
 
-/* Timing: runs nsteps steps with HIP events on the handle's stream around
- * every kernel launch. out[0] = total ms, out[1] = ms in the bulk kernel,
- * out[2] = ms in the near-boundary kernel, out[3] = number of steps. */
+/* Timing: runs nsteps steps with a HIP event on the handle's stream before
+ * the first launch and after every launch. out[0] = total ms, out[1] = ms in
+ * bulk launches (row, cell-tile, in-place or two-step kernels), out[2] = ms
+ * in near-boundary launches, out[3] = number of steps. A launch's time runs
+ * from the previous event, so it includes the gap before the launch. */
 int lbm_time_steps(lbm_domain* d, uint64_t nsteps, double out[4]);
 
 /* Extension: bandwidth of a plain device copy (16-byte vector loads and

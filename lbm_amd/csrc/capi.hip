@@ -130,19 +130,22 @@ int make_plan(lbm_domain* d) {
     p.tb_ntx = (d->L.xl + TB_TX - 1) / TB_TX;
     p.tb_nty = (d->L.yl + TB_TY - 1) / TB_TY;
     const int64_t tiles = static_cast<int64_t>(p.tb_ntx) * p.tb_nty;
-    // By default the z chunks make the grid one round of resident blocks: a
-    // block marches through its planes, so fewer, longer blocks recompute
-    // fewer chunk-boundary planes, and one round leaves no tail.
+    int ncu = 0;
+    if (o->step2 && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d->desc.device) != hipSuccess)
+        ncu = 0;
+    // By default the z chunks make the grid at most one round of resident
+    // blocks: a block marches through its planes, so fewer, longer blocks
+    // recompute fewer chunk-boundary planes, and one round leaves no tail.
     int zc = d->tune_zc;
     if (!zc && o->step2) {
-        int ncu = 0;
-        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d->desc.device) != hipSuccess)
-            ncu = 0;
         const int64_t resident = std::max<int64_t>(1, static_cast<int64_t>(ncu) * o->step2_blocks_per_cu(p.brink));
         const int64_t chunks = std::max<int64_t>(1, resident / tiles);
         zc = static_cast<int>((d->L.zl + chunks - 1) / chunks);
     }
     p.tb_zc = std::max(1, std::min(zc, d->L.zl));
+    // gfx950 deals blocks in turn to its XCDs (32 CUs and one L2 each);
+    // grouping neighbouring tiles on one XCD lets them share halo lines in L2
+    p.tb_nxcd = (ncu >= 32 && ncu % 32 == 0) ? ncu / 32 : 1;
     p.tb_nblocks = tiles * ((d->L.zl + p.tb_zc - 1) / p.tb_zc);
     // (element offsets within a plane are 32-bit in the pass)
     p.pairs = d->tune_pass == 2 && o->step2 != nullptr && !d->aa && d->nb == 0 && !p.flags && p.has_bulk &&
@@ -523,18 +526,12 @@ int lbm_set_tuning(lbm_domain* d, int vec, int block_x) {
     return LBM_OK;
 }
 
-static int run_steps(lbm_domain* d, uint64_t nsteps, hipEvent_t* ev) {
+static int run_steps(lbm_domain* d, uint64_t nsteps, EventLog* log) {
     if (d->aa) {
         for (uint64_t t = 0; t < nsteps; ++t) {
             const StepArgs a = d->args(d->buf[0], d->buf[0], d->materialised ? 1 : 0);
-            hipEvent_t* e4 = ev ? ev + 4 * t : nullptr;
-            if (e4) (void)hipEventRecord(e4[0], d->stream);
             hipError_t e = d->ops->aa_step(a, d->plan, d->aa_phase, d->stream);
-            if (e4) {
-                (void)hipEventRecord(e4[1], d->stream);
-                (void)hipEventRecord(e4[2], d->stream);
-                (void)hipEventRecord(e4[3], d->stream);
-            }
+            if (log) log->mark(d->stream, 0);
             if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e));
             d->aa_phase ^= 1;
             d->materialised = false;
@@ -546,17 +543,14 @@ static int run_steps(lbm_domain* d, uint64_t nsteps, hipEvent_t* ev) {
         for (; t + 2 <= nsteps; t += 2) {
             // (every source is an interior cell: the materialised flag has no effect)
             const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur ^ 1], 0);
-            hipEvent_t* e4 = ev ? ev + 4 * t : nullptr;
-            hipError_t e = d->ops->step2(a, d->plan, d->stream, e4);
-            if (e4)  // the pass's second step: no launch of its own
-                for (int k = 4; k < 8; ++k) (void)hipEventRecord(e4[k], d->stream);
+            hipError_t e = d->ops->step2(a, d->plan, d->stream, log);
             if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("two-step launch: ") + hipGetErrorString(e));
             d->cur ^= 1;
             d->materialised = false;
         }
     for (; t < nsteps; ++t) {
         const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur ^ 1], d->materialised ? 1 : 0);
-        hipError_t e = d->ops->step(a, d->plan, d->stream, ev ? ev + 4 * t : nullptr);
+        hipError_t e = d->ops->step(a, d->plan, d->stream, log);
         if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e));
         d->cur ^= 1;
         d->materialised = false;
@@ -591,19 +585,22 @@ int lbm_time_steps(lbm_domain* d, uint64_t nsteps, double out[4]) {
     if (int rc = check_handle(d)) return rc;
     if (!d->finalized) return fail(LBM_ERR_STATE, "call lbm_finalize first");
     if (!out || nsteps == 0 || nsteps > 100000) return fail(LBM_ERR_ARG, "nsteps in [1, 1e5]");
-    Events ev(4 * nsteps);
+    // one event before the first launch and one after each launch; a launch's
+    // time runs from the previous event (so it includes its launch gap)
+    Events ev(1 + 2 * nsteps);
     HIPCHK(ev.create());
-    if (int rc = run_steps(d, nsteps, ev.data())) return rc;
+    std::vector<int> kind(2 * nsteps, 0);
+    EventLog log{ev.data() + 1, kind.data(), 0, 2 * nsteps};
+    HIPCHK(hipEventRecord(ev[0], d->stream));
+    if (int rc = run_steps(d, nsteps, &log)) return rc;
     HIPCHK(hipStreamSynchronize(d->stream));
     float ms = 0;
     double bulk = 0, near = 0;
-    for (uint64_t t = 0; t < nsteps; ++t) {
-        HIPCHK(hipEventElapsedTime(&ms, ev[4 * t + 0], ev[4 * t + 1]));
-        bulk += ms;
-        HIPCHK(hipEventElapsedTime(&ms, ev[4 * t + 2], ev[4 * t + 3]));
-        near += ms;
+    for (size_t i = 0; i < log.n; ++i) {
+        HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        (kind[i] ? near : bulk) += ms;
     }
-    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[4 * nsteps - 1]));
+    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[log.n]));
     out[0] = ms;
     out[1] = bulk;
     out[2] = near;

@@ -76,10 +76,21 @@ __device__ __forceinline__ void tb_collide(const StepArgs& a, typename Io<M, L>:
                                              static_cast<Calc>(a.tau * a.g[2]));
 }
 
-// Grid: blockIdx.x = tx + ntx * (ty + nty * chunk); chunk c covers planes
+// Logical block of hardware block b when the hardware deals blocks to nxcd
+// L2 domains in turn (b mod nxcd): each domain gets one contiguous range of
+// logical blocks, so tiles that share halo lines share an L2. nxcd <= 1
+// keeps the identity.
+__device__ __forceinline__ int tb_logical_block(int b, int nb, int nxcd) {
+    if (nxcd <= 1) return b;
+    const int x = b % nxcd, slot = b / nxcd;
+    const int per = nb / nxcd, rem = nb % nxcd;
+    return x < rem ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
+}
+
+// Logical block = tx + ntx * (ty + nty * chunk); chunk c covers planes
 // 1 + c * zc .. min((c + 1) * zc, zl).
 template <class L, Mode M, bool BRINK>
-__global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int zc) {
+__global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int zc, int nxcd) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
@@ -89,7 +100,7 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
 
     const FieldLayout& Ly = a.L;
     const int tid = threadIdx.x;
-    int b = blockIdx.x;
+    int b = tb_logical_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x), nxcd);
     const int x0 = 1 + (b % ntx) * TB_TX;
     b /= ntx;
     const int y0 = 1 + (b % nty) * TB_TY;

@@ -18,6 +18,23 @@ struct GhostDims;
 constexpr int TB_TX = 64;
 constexpr int TB_TY = 8;
 
+// Events recorded after kernel launches (lbm_time_steps): mark() records
+// the next event and tags the launch it ends (0: bulk, cell-tile or two-step
+// pass; 1: near-boundary kernel). One record per launch keeps the timing's
+// own cost out of the stream as far as it can.
+struct EventLog {
+    hipEvent_t* ev;
+    int* kind;
+    size_t n, cap;
+    void mark(hipStream_t s, int k) {
+        if (n < cap) {
+            (void)hipEventRecord(ev[n], s);
+            kind[n] = k;
+            ++n;
+        }
+    }
+};
+
 struct StepPlan {
     int V;          // cells per thread along x in k_bulk
     int BX, BY;     // k_bulk block shape (BX * BY == 256)
@@ -36,6 +53,7 @@ struct StepPlan {
     // per block along z.
     bool pairs;
     int tb_ntx, tb_nty, tb_zc;
+    int tb_nxcd;  // L2 domains the blocks are grouped for (1: identity order)
     int64_t tb_nblocks;
 };
 
@@ -46,12 +64,11 @@ struct ModeOps {
     int default_bx;  // threads along x of a k_bulk block when not tuned (0: row width)
     // Supported values of StepPlan::V, 0-terminated.
     int vecs[4];
-    // bulk + near for one step (src/dst in args). ev (may be null) receives
-    // records before/after the bulk and near launches: ev[0..3].
-    hipError_t (*step)(const StepArgs& a, const StepPlan& p, hipStream_t s, hipEvent_t* ev);
-    // two steps in one pass for bulk-only domains (null where not built);
-    // ev as for step
-    hipError_t (*step2)(const StepArgs& a, const StepPlan& p, hipStream_t s, hipEvent_t* ev);
+    // bulk + near for one step (src/dst in args); log (may be null) marks
+    // the end of each launch
+    hipError_t (*step)(const StepArgs& a, const StepPlan& p, hipStream_t s, EventLog* log);
+    // two steps in one pass for bulk-only domains (null where not built)
+    hipError_t (*step2)(const StepArgs& a, const StepPlan& p, hipStream_t s, EventLog* log);
     // blocks of the two-step pass resident per CU (0 where not built)
     int (*step2_blocks_per_cu)(bool brink);
     hipError_t (*get_pdf)(const FieldLayout& Ly, const void* src, double* out, int nx, int ny,

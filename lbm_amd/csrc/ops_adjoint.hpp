@@ -44,7 +44,7 @@ struct AdjOps {
     // field of this mode -> dual field (value, tangent 0)
     hipError_t (*to_dual)(const FieldLayout& Ly, const void* src, void* dst, hipStream_t s);
     // one forward step in dual numbers (plan with V = 1)
-    hipError_t (*dual_step)(const StepArgs& a, const StepPlan& p, hipStream_t s, hipEvent_t* ev);
+    hipError_t (*dual_step)(const StepArgs& a, const StepPlan& p, hipStream_t s, EventLog* log);
     size_t calc_bytes;  // bytes of the mode's arithmetic type (mu buffers)
 };
 

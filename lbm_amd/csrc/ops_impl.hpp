@@ -37,31 +37,24 @@ struct OpsImpl {
     }
 
     template <int... Vs>
-    static hipError_t step_impl(const StepArgs& a, const StepPlan& p, hipStream_t s,
-                                hipEvent_t* ev) {
-        if (ev) (void)hipEventRecord(ev[0], s);
+    static hipError_t step_impl(const StepArgs& a, const StepPlan& p, hipStream_t s, EventLog* log) {
         if (p.cells) {
             hipLaunchKernelGGL((k_cells<L, M>), dim3(static_cast<uint32_t>(p.nblocks_cells)), dim3(256), 0, s, a,
                                p.tpp, p.rxl);
-            if (ev) {
-                (void)hipEventRecord(ev[1], s);
-                (void)hipEventRecord(ev[2], s);
-                (void)hipEventRecord(ev[3], s);
-            }
+            if (log) log->mark(s, 0);
             return hipGetLastError();
         }
         if (p.has_bulk) {
             bool done = (try_bulk<Vs>(a, p, s) || ...);
             if (!done) return hipErrorInvalidValue;
+            if (log) log->mark(s, 0);
         }
-        if (ev) (void)hipEventRecord(ev[1], s);
-        if (ev) (void)hipEventRecord(ev[2], s);
         if (a.nb > 0) {
             const int bs = 128;
             const int64_t nblk = (a.nb + bs - 1) / bs;
             hipLaunchKernelGGL((k_near<L, M>), dim3(static_cast<uint32_t>(nblk)), dim3(bs), 0, s, a);
+            if (log) log->mark(s, 1);
         }
-        if (ev) (void)hipEventRecord(ev[3], s);
         return hipGetLastError();
     }
 
@@ -69,19 +62,16 @@ struct OpsImpl {
     static constexpr bool tb_built() {
         return (M == Mode::F32 || M == Mode::F16 || M == Mode::BF16) && tb_lds_bytes<L, M>() <= 65536;
     }
-    static hipError_t step2_impl(const StepArgs& a, const StepPlan& p, hipStream_t s, hipEvent_t* ev) {
+    static hipError_t step2_impl(const StepArgs& a, const StepPlan& p, hipStream_t s, EventLog* log) {
         if constexpr (tb_built()) {
-            if (ev) (void)hipEventRecord(ev[0], s);
             const dim3 grid(static_cast<uint32_t>(p.tb_nblocks));
             if (p.brink)
-                hipLaunchKernelGGL((k_bulk2<L, M, true>), grid, dim3(256), 0, s, a, p.tb_ntx, p.tb_nty, p.tb_zc);
+                hipLaunchKernelGGL((k_bulk2<L, M, true>), grid, dim3(256), 0, s, a, p.tb_ntx, p.tb_nty, p.tb_zc,
+                                   p.tb_nxcd);
             else
-                hipLaunchKernelGGL((k_bulk2<L, M, false>), grid, dim3(256), 0, s, a, p.tb_ntx, p.tb_nty, p.tb_zc);
-            if (ev) {
-                (void)hipEventRecord(ev[1], s);
-                (void)hipEventRecord(ev[2], s);
-                (void)hipEventRecord(ev[3], s);
-            }
+                hipLaunchKernelGGL((k_bulk2<L, M, false>), grid, dim3(256), 0, s, a, p.tb_ntx, p.tb_nty, p.tb_zc,
+                                   p.tb_nxcd);
+            if (log) log->mark(s, 0);
             return hipGetLastError();
         } else {
             return hipErrorInvalidValue;

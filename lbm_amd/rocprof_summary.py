@@ -2,7 +2,7 @@
 
     python -m lbm_amd.rocprof_summary stats <kernel-trace results.db>
     python -m lbm_amd.rocprof_summary traffic <FETCH_SIZE results.db> <WRITE_SIZE results.db> \
-        --kernel-match k_bulk --mode-match "Mode)1" --alg-bytes N --copy-bytes B
+        --kernel-match k_bulk< --mode-match "Mode)1" --alg-bytes N --copy-bytes B
 
 `stats` lists per-kernel calls and average duration (the --stats view).
 `traffic` scales the FETCH_SIZE / WRITE_SIZE counters (KiB per dispatch) of
@@ -46,7 +46,7 @@ def main(argv=None):
     t = sub.add_parser("traffic")
     t.add_argument("fetch_db")
     t.add_argument("write_db")
-    t.add_argument("--kernel-match", default="k_bulk")
+    t.add_argument("--kernel-match", default="k_bulk<")
     t.add_argument("--mode-match", default="")
     t.add_argument("--alg-bytes", type=float, required=True)
     t.add_argument("--copy-bytes", type=float, required=True)
