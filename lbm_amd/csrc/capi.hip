@@ -147,9 +147,10 @@ int make_plan(lbm_domain* d) {
     // grouping neighbouring tiles on one XCD lets them share halo lines in L2
     p.tb_nxcd = (ncu >= 32 && ncu % 32 == 0) ? ncu / 32 : 1;
     p.tb_nblocks = tiles * ((d->L.zl + p.tb_zc - 1) / p.tb_zc);
-    // (element offsets within a plane are 32-bit in the pass)
+    // (byte offsets within a plane are 32-bit in the pass)
     p.pairs = d->tune_pass == 2 && o->step2 != nullptr && !d->aa && d->nb == 0 && !p.flags && p.has_bulk &&
-              p.tb_nblocks <= 0x7fffffffLL && d->L.plane < (int64_t{1} << 32);
+              p.tb_nblocks <= 0x7fffffffLL &&
+              d->L.plane * static_cast<int64_t>(d->esize) < (int64_t{1} << 32);
     return LBM_OK;
 }
 

@@ -67,6 +67,17 @@ constexpr size_t tb_lds_bytes() {
     return static_cast<size_t>(TbSlots<L>::NSLOT) * TB_CELLS * sizeof(typename Io<M, L>::T);
 }
 
+// base[byte_off]: a uniform base and a 32-bit byte offset, the form of a
+// global access with a scalar base and one offset register per lane.
+template <class T>
+__device__ __forceinline__ const T& tb_at(const T* base, uint32_t byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <class T>
+__device__ __forceinline__ T* tb_ptr(T* base, uint32_t byte_off) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
+}
+
 template <class L, Mode M, bool BRINK>
 __device__ __forceinline__ void tb_collide(const StepArgs& a, typename Io<M, L>::Calc (&f)[L::Q],
                                            typename Io<M, L>::Calc alpha) {
@@ -110,8 +121,9 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
     const T* __restrict__ src = static_cast<const T*>(a.src);
     T* __restrict__ dst = static_cast<T*>(a.dst);
     const Calc* __restrict__ alpha = static_cast<const Calc*>(a.alpha);
-    // every element offset of a plane fits 32 bits (the host checks)
-    const uint32_t py = static_cast<uint32_t>(Ly.px), pz = static_cast<uint32_t>(Ly.pxy);
+    // byte offsets within a plane fit 32 bits (the host checks plane bytes)
+    constexpr uint32_t TS = sizeof(T);
+    const uint32_t py = static_cast<uint32_t>(Ly.px) * TS, pz = static_cast<uint32_t>(Ly.pxy) * TS;
 
     // this thread's patch cells: tile-local (pi, pj) in [-1, TX] x [-1, TY]
     int pi[TB_PASS], pj[TB_PASS], gx[TB_PASS], gy[TB_PASS];
@@ -129,17 +141,21 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
 
     T hold[TB_PASS][Q];
     for (int zs = z0 - 1; zs <= z1 + 1; ++zs) {
+        // The plane stride is redefined in each iteration (an empty asm that
+        // emits nothing): the Q plane bases are then recomputed with a few
+        // scalar instructions where they are used instead of being hoisted
+        // out of the loop, where 2 Q of them overflow the scalar registers.
+        int64_t plane = Ly.plane;
+        asm volatile("" : "+s"(plane));
         // (1) step 1 on plane zs
         const int gz = wrap1(zs, Ly.zl);
-        const uint32_t zs3[3] = {static_cast<uint32_t>(Ly.off) + pz * wrap1(gz - 1, Ly.zl),
-                                 static_cast<uint32_t>(Ly.off) + pz * gz,
-                                 static_cast<uint32_t>(Ly.off) + pz * wrap1(gz + 1, Ly.zl)};
+        const uint32_t off = static_cast<uint32_t>(Ly.off) * TS;
+        const uint32_t zs3[3] = {off + pz * wrap1(gz - 1, Ly.zl), off + pz * gz, off + pz * wrap1(gz + 1, Ly.zl)};
 #pragma unroll
         for (int k = 0; k < TB_PASS; ++k) {
             if (!live[k]) continue;
             // 32-bit element offsets within a plane (the plane base is uniform)
-            const uint32_t xs[3] = {static_cast<uint32_t>(wrap1(gx[k] - 1, Ly.xl)), static_cast<uint32_t>(gx[k]),
-                                    static_cast<uint32_t>(wrap1(gx[k] + 1, Ly.xl))};
+            const uint32_t xs[3] = {TS * wrap1(gx[k] - 1, Ly.xl), TS * gx[k], TS * wrap1(gx[k] + 1, Ly.xl)};
             const uint32_t ys[3] = {py * wrap1(gy[k] - 1, Ly.yl), py * gy[k], py * wrap1(gy[k] + 1, Ly.yl)};
             Calc f[Q];
             static_for<0, Q>([&](auto qc) {
@@ -147,10 +163,10 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                 // source x - c_q: index 0 for c = +1, 1 for 0, 2 for -1
                 constexpr int ix = 1 - L::cx(q), iy = 1 - L::cy(q), iz = 1 - L::cz(q);
                 const uint32_t o = zs3[iz] + ys[iy] + xs[ix];
-                f[q] = IO::template to_calc<q>(src[q * Ly.plane + o]);
+                f[q] = IO::template to_calc<q>(tb_at(src + q * plane, o));
             });
             Calc al = Calc(0);
-            if constexpr (BRINK) al = alpha[zs3[1] + ys[1] + xs[1]];
+            if constexpr (BRINK) al = alpha[(zs3[1] + ys[1] + xs[1]) / TS];
             tb_collide<L, M, BRINK>(a, f, al);
             static_for<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
@@ -178,13 +194,13 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                         constexpr int q = decltype(qc)::value;
                         f[q] = IO::template to_calc<q>(lds[S::template slot<q>(zo)][c]);
                     });
-                    const uint32_t e = static_cast<uint32_t>(Ly.off) + pz * zo + py * oy + ox;
+                    const uint32_t eb = TS * (static_cast<uint32_t>(Ly.off) + ox) + pz * zo + py * oy;
                     Calc al = Calc(0);
-                    if constexpr (BRINK) al = alpha[e];
+                    if constexpr (BRINK) al = alpha[eb / TS];
                     tb_collide<L, M, BRINK>(a, f, al);
                     static_for<0, Q>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
-                        store_stream1<T>(dst + q * Ly.plane + e, IO::template from_calc<q>(f[q]));
+                        store_stream1<T>(tb_ptr(dst + q * plane, eb), IO::template from_calc<q>(f[q]));
                     });
                 }
             }
