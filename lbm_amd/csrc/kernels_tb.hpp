@@ -16,8 +16,9 @@
 //       store g2.
 // A value sent with cz = -1 is read in the same iteration, cz = 0 in the next
 // and cz = +1 in the one after. The cz >= 0 values are held in registers
-// across (2) and written afterwards, so LDS holds one plane of slots for the
-// cz = -1 and cz = 0 directions and two planes for cz = +1.
+// across (2), and the cz = +1 values one plane longer, and written after (2)
+// of the iteration before they are read, so LDS holds one plane of slots per
+// direction: Q x TX x TY values.
 //
 // Coordinates are kept unwrapped (x0 - 1 .. x0 + TX, zs = z0 - 1 .. z1 + 1)
 // for the LDS bookkeeping and wrapped onto the periodic box only to address
@@ -46,19 +47,18 @@ struct TbSlots {
         return n;
     }
     static constexpr int NM = count(-1), N0 = count(0), NP = count(1);
-    static constexpr int NSLOT = NM + N0 + 2 * NP;
+    static constexpr int NSLOT = NM + N0 + NP;
     static constexpr int rank(int q) {
         int r = 0;
         for (int k = 0; k < q; ++k) r += (L::cz(k) == L::cz(q)) ? 1 : 0;
         return r;
     }
     static constexpr int base(int q) {
-        return L::cz(q) < 0 ? rank(q) : (L::cz(q) == 0 ? NM + rank(q) : NM + N0 + 2 * rank(q));
+        return L::cz(q) < 0 ? rank(q) : (L::cz(q) == 0 ? NM + rank(q) : NM + N0 + rank(q));
     }
     template <int q>
-    static __device__ __forceinline__ int slot(int zd) {
-        if constexpr (L::cz(q) > 0) return base(q) + (zd & 1);
-        else return base(q);
+    static __device__ __forceinline__ int slot(int) {
+        return base(q);
     }
 };
 
@@ -140,6 +140,7 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
     }
 
     T hold[TB_PASS][Q];
+    T prev[TB_PASS][Q];  // c_z = +1 values of the previous plane (only those are used)
     for (int zs = z0 - 1; zs <= z1 + 1; ++zs) {
         // The plane stride is redefined in each iteration (an empty asm that
         // emits nothing): the Q plane bases are then recomputed with a few
@@ -206,7 +207,8 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
             }
         }
         __syncthreads();
-        // sends with cz >= 0, read by step 2 of the next two iterations
+        // sends read by step 2 of the next iteration: c_z = 0 of this plane,
+        // c_z = +1 of the previous plane (held one more plane in registers)
 #pragma unroll
         for (int k = 0; k < TB_PASS; ++k) {
             if (!live[k]) continue;
@@ -215,8 +217,9 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                 if constexpr (L::cz(q) >= 0) {
                     const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
                     if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY)
-                        lds[S::template slot<q>(zs + L::cz(q))][dj * TB_TX + di] = hold[k][q];
+                        lds[S::template slot<q>(zs)][dj * TB_TX + di] = L::cz(q) == 0 ? hold[k][q] : prev[k][q];
                 }
+                if constexpr (L::cz(q) > 0) prev[k][q] = hold[k][q];
             });
         }
     }

@@ -21,9 +21,8 @@ from tests.lbm_math import tgv_fields
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(1, 1, 1), (2, 3, 5), (3, 1, 2), (65, 9, 17), (130, 17, 33), (64, 8, 4), (200, 40, 24)]
-# LDS slots of the pass fit 64 KiB for these (lattice, storage) pairs
-BUILT = {(15, "f32"), (19, "f32"), (15, "f16"), (19, "f16"), (27, "f16"), (15, "bf16"), (19, "bf16"),
-         (27, "bf16")}
+# the pass is built for every lattice in 4- and 2-byte storage (its LDS slots fit 64 KiB)
+BUILT = {(q, s) for q in (15, 19, 27) for s in ("f32", "f16", "bf16")}
 
 
 def say(*a):
