@@ -542,8 +542,9 @@ static int run_steps(lbm_domain* d, uint64_t nsteps, EventLog* log) {
     uint64_t t = 0;
     if (d->plan.pairs)
         for (; t + 2 <= nsteps; t += 2) {
-            // (every source is an interior cell: the materialised flag has no effect)
-            const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur ^ 1], 0);
+            // (every source is an interior cell, so the flag has no effect here;
+            // it is passed as a single step would take it)
+            const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur ^ 1], d->materialised ? 1 : 0);
             hipError_t e = d->ops->step2(a, d->plan, d->stream, log);
             if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("two-step launch: ") + hipGetErrorString(e));
             d->cur ^= 1;
