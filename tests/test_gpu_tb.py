@@ -120,3 +120,25 @@ def test_two_step_pass_256_cube(storage):
     assert np.array_equal(out[1][0].view(np.uint64), out[2][0].view(np.uint64))
     assert np.array_equal(out[1][1].view(np.uint64), out[2][1].view(np.uint64))
     assert np.array_equal(out[1][2], out[2][2])
+
+
+def test_time_steps_accounts_every_launch():
+    """lbm_time_steps records one event before the first launch and one after
+    each launch: the bulk and near intervals add up to the total, a
+    bulk-only box (two-step passes plus one single step) has no near time,
+    and the row kernel with its near-boundary launch has some."""
+    say("time_steps accounting")
+    per = capi.Domain(70, 12, 9, 0.8, q=19, storage="f32", periodic=(1, 1, 1))
+    per.finalize()
+    t = per.time_steps(7)
+    per.close()
+    assert t["steps"] == 7 and t["near_ms"] == 0.0
+    assert t["total_ms"] > 0 and abs(t["bulk_ms"] - t["total_ms"]) <= 1e-3 * t["total_ms"]
+    cav = capi.Domain(70, 12, 9, 0.8, q=19, storage="f32")
+    scenario.cavity(70, 12, 9).apply(cav)
+    cav.finalize()
+    cav.set_tuning(2, 0)  # row kernel + near-boundary kernel
+    t = cav.time_steps(5)
+    cav.close()
+    assert t["steps"] == 5 and t["near_ms"] > 0 and t["bulk_ms"] > 0
+    assert abs(t["bulk_ms"] + t["near_ms"] - t["total_ms"]) <= 1e-3 * t["total_ms"]
