@@ -73,6 +73,9 @@ struct DeviceOptions {
     int device = 0;
     std::array<int, 3> periodic{{0, 0, 0}};
     int layout = LBM_LAYOUT_AB;  // LBM_LAYOUT_AA: one buffer updated in place
+    // 2: where the domain allows it (bulk-only, fp32/fp16/bf16), step(n) runs
+    // pairs of steps as one pass over the field, bit-identical to single steps
+    int steps_per_pass = 2;
 };
 
 template <class M>
@@ -249,6 +252,7 @@ public:
         d.spacing[1] = yspacing;
         d.spacing[2] = zspacing;
         check(lbm_domain_create(&d, &h_));
+        check(lbm_set_time_blocking(h_, opt.steps_per_pass, 0));
         fluid_.assign((xl + 2) * (yl + 2) * (zl + 2), 1);
     }
     ~Domain() { lbm_domain_destroy(h_); }

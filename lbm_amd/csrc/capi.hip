@@ -808,9 +808,17 @@ int lbm_set_time_blocking(lbm_domain* d, int steps_per_pass, int z_chunk) {
     if (!d) return fail(LBM_ERR_ARG, "null domain handle");
     if (steps_per_pass != 1 && steps_per_pass != 2) return fail(LBM_ERR_ARG, "steps_per_pass must be 1 or 2");
     if (z_chunk < 0) return fail(LBM_ERR_ARG, "z_chunk must be >= 0");
+    const int op = d->tune_pass, oz = d->tune_zc;
     d->tune_pass = steps_per_pass;
     d->tune_zc = z_chunk;
-    if (d->finalized) return make_plan(d);
+    if (d->finalized) {
+        if (int rc = make_plan(d)) {
+            d->tune_pass = op;
+            d->tune_zc = oz;
+            make_plan(d);
+            return rc;
+        }
+    }
     return LBM_OK;
 }
 
