@@ -133,14 +133,26 @@ int make_plan(lbm_domain* d) {
     int ncu = 0;
     if (o->step2 && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d->desc.device) != hipSuccess)
         ncu = 0;
-    // By default the z chunks make the grid at most one round of resident
-    // blocks: a block marches through its planes, so fewer, longer blocks
-    // recompute fewer chunk-boundary planes, and one round leaves no tail.
+    // Default z chunks: a block marches through its zc planes plus one more
+    // at each end, so the pass takes about rounds x (zc + 2) plane times,
+    // with rounds = ceil(blocks / resident blocks). The chunk count that
+    // minimises it is taken, among chunks of at most 32 planes (longer ones
+    // measured slower on planes of many tiles).
     int zc = d->tune_zc;
     if (!zc && o->step2) {
         const int64_t resident = std::max<int64_t>(1, static_cast<int64_t>(ncu) * o->step2_blocks_per_cu(p.brink));
-        const int64_t chunks = std::max<int64_t>(1, resident / tiles);
-        zc = static_cast<int>((d->L.zl + chunks - 1) / chunks);
+        const int64_t zl = d->L.zl;
+        int64_t best = -1, best_cost = 0;
+        for (int64_t c = (zl + 31) / 32; c <= zl; ++c) {
+            const int64_t z = (zl + c - 1) / c;
+            if (c > 1 && (zl + c - 2) / (c - 1) == z) continue;  // same chunk length as c - 1
+            const int64_t cost = ((c * tiles + resident - 1) / resident) * (z + 2);
+            if (best < 0 || cost < best_cost) {
+                best = c;
+                best_cost = cost;
+            }
+        }
+        zc = static_cast<int>((zl + best - 1) / best);
     }
     p.tb_zc = std::max(1, std::min(zc, d->L.zl));
     // gfx950 deals blocks in turn to its XCDs (32 CUs and one L2 each);
