@@ -324,9 +324,9 @@ int lbm_set_tuning(lbm_domain* d, int vec, int block_x);
 
 /* Time steps per pass over the field. Where every interior cell is bulk
  * (periodic along all three axes, no boundary cells) and the storage mode is
- * fp32, fp16 or bf16, lbm_step can run two steps in one pass: the field is
- * read once and written once for both, with the intermediate state held on
- * chip. Results are the same bits as two single steps. steps_per_pass: 1 or 2
+ * fp32, fp16 or bf16 (16-bit modes: x rows longer than 32 cells), lbm_step
+ * can run two steps in one pass: the field is read once and written once for
+ * both, with the intermediate state held on chip. Results are the same bits as two single steps. steps_per_pass: 1 or 2
  * (default 2; a domain where the pass does not apply steps singly).
  * z_chunk: planes per block of the pass (0 = automatic). No reference
  * counterpart: a scheduling choice of this implementation. */

@@ -159,10 +159,14 @@ int make_plan(lbm_domain* d) {
     // grouping neighbouring tiles on one XCD lets them share halo lines in L2
     p.tb_nxcd = (ncu >= 32 && ncu % 32 == 0) ? ncu / 32 : 1;
     p.tb_nblocks = tiles * ((d->L.zl + p.tb_zc - 1) / p.tb_zc);
-    // (byte offsets within a plane are 32-bit in the pass)
+    // (byte offsets within a plane are 32-bit in the pass.) The 16-bit pass
+    // is bound by instruction issue: on rows of 32 cells or fewer most lanes
+    // of its 64-cell tile rows idle, and single steps (cell tiles over packed
+    // rows) measured faster there.
     p.pairs = d->tune_pass == 2 && o->step2 != nullptr && !d->aa && d->nb == 0 && !p.flags && p.has_bulk &&
               p.tb_nblocks <= 0x7fffffffLL &&
-              d->L.plane * static_cast<int64_t>(d->esize) < (int64_t{1} << 32);
+              d->L.plane * static_cast<int64_t>(d->esize) < (int64_t{1} << 32) &&
+              (d->esize >= 4 || d->L.xl > 32);
     return LBM_OK;
 }
 
