@@ -56,6 +56,7 @@ struct BatchArgs {
     int64_t nnear;
     int parity;        // src = buf[parity], dst = buf[parity ^ 1]
     int materialised;  // the step pulls stored boundary values
+    int freeslip;      // some domain has a FreeSlip condition (cell tiles built with its rule)
 };
 
 struct BatchOps {

@@ -121,6 +121,8 @@ int make_plan(lbm_domain* d) {
     p.tpp = static_cast<int>((plane_cells + 63) / 64);
     p.nblocks_cells = (static_cast<int64_t>(p.tpp) * d->L.zl + 3) / 4;
     p.rxl = 1.0 / static_cast<double>(d->L.xl);
+    p.freeslip = false;
+    for (const HostCond& c : d->conds) p.freeslip |= c.kind == LBM_BC_FREESLIP;
     p.cells = d->tune_vec == 0 && (d->nb > 0 || d->compact) &&
               p.nblocks_cells <= 0x7fffffffLL && (plane_cells + 63) / 64 <= 0x7fffffffLL;
     if (!p.cells && d->compact && V > 1)

@@ -31,6 +31,7 @@ struct lbm_batch {
     int V = 0, bpc = 8, bx_policy = 0;
     int kernel = BK_CELLS;      // enum BatchKernel
     bool any_compact = false;   // some domain packs its rows: row kernels take V = 1 only
+    bool any_freeslip = false;  // some domain has a FreeSlip condition
     int64_t near_cells = 0;     // NEAR cells of all domains
     BatchView* d_views = nullptr;
     TileRef* d_tiles = nullptr;
@@ -157,6 +158,7 @@ hipError_t enqueue(lbm_batch* b, uint64_t nsteps, int parity, int mat) {
     a.ntiles = b->ntiles;
     a.near = b->d_near;
     a.nnear = b->nnear;
+    a.freeslip = b->any_freeslip ? 1 : 0;
     // blocks that give every tile its own block (cell tiles: 4 wavefronts a block)
     const int64_t full = std::max<int64_t>(1, b->kernel == BK_CELLS ? (b->ntiles + 3) / 4 : b->ntiles);
     const int grid_bulk = static_cast<int>(b->bpc ? std::min<int64_t>(full, static_cast<int64_t>(b->ncu) * b->bpc)
@@ -214,6 +216,7 @@ int lbm_batch_create(lbm_domain* const* domains, int n, lbm_batch** out) {
     for (const lbm_domain* d : b->doms) {
         b->ncells += static_cast<uint64_t>(d->L.xl) * d->L.yl * d->L.zl;
         b->any_compact |= d->compact;
+        b->any_freeslip |= d->plan.freeslip;
     }
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         delete b;

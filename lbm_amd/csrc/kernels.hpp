@@ -345,8 +345,9 @@ constexpr int dir_of(int cx, int cy, int cz) {
 // the sums of x's own post-collision values and own(k) returns own value k.
 // Sets stale when the rule writes nothing (FreeSlip with no fluid candidate);
 // the value returned is then b's stored value, which is what the reference
-// reads too.
-template <class L, Mode M, int q, class OwnFn>
+// reads too. FS = false builds the rule without the FreeSlip case, for
+// domains that have no FreeSlip condition.
+template <class L, Mode M, int q, bool FS = true, class OwnFn>
 __device__ __forceinline__ typename Io<M, L>::Calc pull_boundary(
     const StepArgs& a, const Cond& cd, const OwnSums<typename Io<M, L>::Calc>& os, OwnFn own, int x,
     int y, int z, int sx, int sy, int sz, int64_t se, bool& stale) {
@@ -357,29 +358,31 @@ __device__ __forceinline__ typename Io<M, L>::Calc pull_boundary(
     const FieldLayout& Ly = a.L;
     const T* __restrict__ src = static_cast<const T*>(a.src);
     stale = false;
-    if (cd.kind == BC_FREESLIP) {  // boundary.hpp:91-110 with b = (sx,sy,sz), b + c_q = x
-        // Candidates in the reference's order; candidate j replaces the
-        // coordinates flagged in kFsSwap[j] by x's and reflects those
-        // velocity components. The reflected directions are compile-time.
-        typename IO::Calc v{};
-        bool found = false;
-        static_for<0, 6>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            constexpr int rq = dir_of<L>(kFsSwap[j][0] ? -cx : cx, kFsSwap[j][1] ? -cy : cy,
-                                         kFsSwap[j][2] ? -cz : cz);
-            static_assert(rq >= 0, "reflected velocity outside the lattice");
-            if (!found) {
-                const int64_t ce = cell_elem(Ly, kFsSwap[j][0] ? x : sx, kFsSwap[j][1] ? y : sy,
-                                             kFsSwap[j][2] ? z : sz);
-                if (a.kind[ce] < KIND_COND0) {
-                    v = IO::template to_calc<rq>(src[rq * Ly.plane + ce]);
-                    found = true;
+    if constexpr (FS) {
+        if (cd.kind == BC_FREESLIP) {  // boundary.hpp:91-110 with b = (sx,sy,sz), b + c_q = x
+            // Candidates in the reference's order; candidate j replaces the
+            // coordinates flagged in kFsSwap[j] by x's and reflects those
+            // velocity components. The reflected directions are compile-time.
+            typename IO::Calc v{};
+            bool found = false;
+            static_for<0, 6>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                constexpr int rq = dir_of<L>(kFsSwap[j][0] ? -cx : cx, kFsSwap[j][1] ? -cy : cy,
+                                             kFsSwap[j][2] ? -cz : cz);
+                static_assert(rq >= 0, "reflected velocity outside the lattice");
+                if (!found) {
+                    const int64_t ce = cell_elem(Ly, kFsSwap[j][0] ? x : sx, kFsSwap[j][1] ? y : sy,
+                                                 kFsSwap[j][2] ? z : sz);
+                    if (a.kind[ce] < KIND_COND0) {
+                        v = IO::template to_calc<rq>(src[rq * Ly.plane + ce]);
+                        found = true;
+                    }
                 }
-            }
-        });
-        if (found) return v;
-        stale = true;
-        return IO::template to_calc<q>(src[q * Ly.plane + se]);
+            });
+            if (found) return v;
+            stale = true;
+            return IO::template to_calc<q>(src[q * Ly.plane + se]);
+        }
     }
     if (cd.kind < BC_NOSLIP || cd.kind > BC_PRESSURE) {
         stale = true;
@@ -389,13 +392,13 @@ __device__ __forceinline__ typename Io<M, L>::Calc pull_boundary(
 }
 
 // The same with x's own values already in registers.
-template <class L, Mode M, int q>
+template <class L, Mode M, int q, bool FS = true>
 __device__ __forceinline__ typename Io<M, L>::Calc pull_nonfluid(
     const StepArgs& a, const Cond& cd, const typename Io<M, L>::Calc (&own)[L::Q], int x, int y,
     int z, int sx, int sy, int sz, int64_t se, bool& stale) {
     using Calc = typename Io<M, L>::Calc;
     const OwnSums<Calc> os = own_sums<L, Calc>(own);
-    return pull_boundary<L, M, q>(a, cd, os, [&](auto kc) { return own[decltype(kc)::value]; }, x, y, z,
+    return pull_boundary<L, M, q, FS>(a, cd, os, [&](auto kc) { return own[decltype(kc)::value]; }, x, y, z,
                                   sx, sy, sz, se, stale);
 }
 
@@ -467,7 +470,7 @@ __device__ __forceinline__ void near_cell(const StepArgs& a, int x, int y, int z
 // then replaces the values pulled from non-fluid sources by their boundary
 // values (as near_cell), and all lanes collide and store together. The bits
 // equal those of bulk_cells / near_cell for the same cell.
-template <class L, Mode M>
+template <class L, Mode M, bool FS = true>
 __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z) {
     using IO = Io<M, L>;
     using T = typename IO::T;
@@ -523,7 +526,7 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
             const uint8_t sk = a.kind[se];
             if (sk >= KIND_COND0) {
                 bool stale;
-                f[q] = pull_boundary<L, M, q>(a, a.conds[sk - KIND_COND0], os, own, x, y, z, sx, sy, sz, se,
+                f[q] = pull_boundary<L, M, q, FS>(a, a.conds[sk - KIND_COND0], os, own, x, y, z, sx, sy, sz, se,
                                               stale);
             }
         });
@@ -549,7 +552,7 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
 
 // Cell-tile kernel: wavefront w of the grid takes the 64 cells
 // 64 (w mod tpp) .. + 63 (x fastest) of the z plane 1 + w / tpp.
-template <class L, Mode M>
+template <class L, Mode M, bool FS = true>
 __global__ void __launch_bounds__(256) k_cells(StepArgs a, int tpp, double rxl) {
     const int64_t w = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int z = 1 + static_cast<int>(w / tpp);
@@ -567,7 +570,7 @@ __global__ void __launch_bounds__(256) k_cells(StepArgs a, int tpp, double rxl) 
         ++y;
         x -= xl;
     }
-    any_cell<L, M>(a, 1 + static_cast<int>(x), 1 + static_cast<int>(y), z);
+    any_cell<L, M, FS>(a, 1 + static_cast<int>(x), 1 + static_cast<int>(y), z);
 }
 
 // Near-boundary kernel: one thread per listed NEAR cell.

@@ -237,7 +237,8 @@ __device__ __forceinline__ void adj_cell(const AdjArgs& a, int x, int y, int z, 
             h[q] = IO::template to_calc<q>(gp[q * Ly.plane + se]);
         } else {
             bool stale;
-            h[q] = pull_nonfluid<L, M, q>(sa, a.conds[k - KIND_COND0], own, x, y, z, sx, sy, sz, se, stale);
+            // (the adjoint takes no FreeSlip condition)
+            h[q] = pull_nonfluid<L, M, q, false>(sa, a.conds[k - KIND_COND0], own, x, y, z, sx, sy, sz, se, stale);
         }
     });
 
@@ -374,7 +375,8 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
             const uint8_t k = a.kind[se];
             if (k >= KIND_COND0) {
                 bool stale;
-                h[q] = pull_boundary<L, M, q>(sa, a.conds[k - KIND_COND0], os, own, x, y, z, sx, sy, sz, se, stale);
+                h[q] = pull_boundary<L, M, q, false>(sa, a.conds[k - KIND_COND0], os, own, x, y, z, sx, sy, sz, se,
+                                                     stale);
             }
         });
     }

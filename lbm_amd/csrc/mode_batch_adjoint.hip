@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) k_badj_recompute(BatchAdjArgs b) {
         a.materialised = b.materialised;
         a.tau = v.tau;
         a.omega = v.omega;
-        any_cell<L, M>(a, x, y, z);
+        any_cell<L, M, false>(a, x, y, z);  // the adjoint takes no FreeSlip condition
     }
 }
 

@@ -39,8 +39,12 @@ struct OpsImpl {
     template <int... Vs>
     static hipError_t step_impl(const StepArgs& a, const StepPlan& p, hipStream_t s, EventLog* log) {
         if (p.cells) {
-            hipLaunchKernelGGL((k_cells<L, M>), dim3(static_cast<uint32_t>(p.nblocks_cells)), dim3(256), 0, s, a,
-                               p.tpp, p.rxl);
+            if (p.freeslip)
+                hipLaunchKernelGGL((k_cells<L, M, true>), dim3(static_cast<uint32_t>(p.nblocks_cells)), dim3(256), 0,
+                                   s, a, p.tpp, p.rxl);
+            else
+                hipLaunchKernelGGL((k_cells<L, M, false>), dim3(static_cast<uint32_t>(p.nblocks_cells)), dim3(256), 0,
+                                   s, a, p.tpp, p.rxl);
             if (log) log->mark(s, 0);
             return hipGetLastError();
         }
