@@ -38,7 +38,7 @@ constexpr int TB_CELLS = TB_TX * TB_TY;
 constexpr int TB_OUT = TB_CELLS / 256;          // step-2 cells per thread
 static_assert(TB_CELLS % 256 == 0, "step 2 gives every thread the same number of cells");
 
-// LDS slot of direction q for the destination plane zd.
+// LDS slot row of direction q (one plane per direction, see above).
 template <class L>
 struct TbSlots {
     static constexpr int count(int cz) {
@@ -57,7 +57,7 @@ struct TbSlots {
         return L::cz(q) < 0 ? rank(q) : (L::cz(q) == 0 ? NM + rank(q) : NM + N0 + rank(q));
     }
     template <int q>
-    static __device__ __forceinline__ int slot(int) {
+    static __device__ __forceinline__ constexpr int slot() {
         return base(q);
     }
 };
@@ -140,7 +140,9 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
     }
 
     T hold[TB_PASS][Q];
-    T prev[TB_PASS][Q];  // c_z = +1 values of the previous plane (only those are used)
+    // c_z = +1 values of the previous plane (only those are used); the first
+    // iteration sends them for plane z0 - 1, which no step 2 reads
+    T prev[TB_PASS][Q] = {};
     for (int zs = z0 - 1; zs <= z1 + 1; ++zs) {
         // The plane stride is redefined in each iteration (an empty asm that
         // emits nothing): the Q plane bases are then recomputed with a few
@@ -155,7 +157,7 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
 #pragma unroll
         for (int k = 0; k < TB_PASS; ++k) {
             if (!live[k]) continue;
-            // 32-bit element offsets within a plane (the plane base is uniform)
+            // 32-bit byte offsets within a plane (the plane base is uniform)
             const uint32_t xs[3] = {TS * wrap1(gx[k] - 1, Ly.xl), TS * gx[k], TS * wrap1(gx[k] + 1, Ly.xl)};
             const uint32_t ys[3] = {py * wrap1(gy[k] - 1, Ly.yl), py * gy[k], py * wrap1(gy[k] + 1, Ly.yl)};
             Calc f[Q];
@@ -175,7 +177,7 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                 if constexpr (L::cz(q) < 0) {
                     const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
                     if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY)
-                        lds[S::template slot<q>(zs - 1)][dj * TB_TX + di] = v;
+                        lds[S::template slot<q>()][dj * TB_TX + di] = v;
                 } else {
                     hold[k][q] = v;
                 }
@@ -193,7 +195,7 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                     Calc f[Q];
                     static_for<0, Q>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
-                        f[q] = IO::template to_calc<q>(lds[S::template slot<q>(zo)][c]);
+                        f[q] = IO::template to_calc<q>(lds[S::template slot<q>()][c]);
                     });
                     const uint32_t eb = TS * (static_cast<uint32_t>(Ly.off) + ox) + pz * zo + py * oy;
                     Calc al = Calc(0);
@@ -217,7 +219,7 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                 if constexpr (L::cz(q) >= 0) {
                     const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
                     if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY)
-                        lds[S::template slot<q>(zs)][dj * TB_TX + di] = L::cz(q) == 0 ? hold[k][q] : prev[k][q];
+                        lds[S::template slot<q>()][dj * TB_TX + di] = L::cz(q) == 0 ? hold[k][q] : prev[k][q];
                 }
                 if constexpr (L::cz(q) > 0) prev[k][q] = hold[k][q];
             });
