@@ -107,7 +107,9 @@ def test_pass_applies_only_to_bulk_only_fp32_fp16_bf16_domains():
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 def test_two_step_pass_256_cube(storage):
     """Benchmark size (D3Q19 256^3, Taylor-Green vortex with density noise,
-    tau = 0.8): 10 steps in pairs equal 10 single steps bit for bit."""
+    tau = 0.8): 201 steps (100 passes and one single step) equal 201 single
+    steps bit for bit. A race between the pass's LDS sends and reads would
+    show here as a sporadic difference."""
     n = 256
     say(f"two-step pass 256^3 {storage}")
     rho, u = tgv_fields(n, n, n, u0=0.02)
@@ -117,7 +119,7 @@ def test_two_step_pass_256_cube(storage):
         d.finalize()
         d.set_time_blocking(spp)
         d.init_equilibrium(rho, u)
-        d.step(10)
+        d.step(201)
         r, v = d.moments()
         out[spp] = (r, v, d.conservation())
         d.close()
