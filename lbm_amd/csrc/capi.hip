@@ -529,7 +529,7 @@ int lbm_finalize(lbm_domain* d) {
 }
 
 int lbm_set_tuning(lbm_domain* d, int vec, int block_x) {
-    if (!d) return fail(LBM_ERR_ARG, "null domain handle");
+    if (int rc = check_handle(d)) return rc;  // (planning queries the domain's device)
     if (d->aa && vec > 0) return fail(LBM_ERR_UNSUPPORTED, "the in-place layout steps with cell tiles only");
     const int ov = d->tune_vec, ob = d->tune_bx;
     d->tune_vec = vec;
@@ -823,7 +823,7 @@ int lbm_get_info(lbm_domain* d, lbm_info* info) {
 }
 
 int lbm_set_time_blocking(lbm_domain* d, int steps_per_pass, int z_chunk) {
-    if (!d) return fail(LBM_ERR_ARG, "null domain handle");
+    if (int rc = check_handle(d)) return rc;  // (planning queries the domain's device)
     if (steps_per_pass != 1 && steps_per_pass != 2) return fail(LBM_ERR_ARG, "steps_per_pass must be 1 or 2");
     if (z_chunk < 0) return fail(LBM_ERR_ARG, "z_chunk must be >= 0");
     const int op = d->tune_pass, oz = d->tune_zc;
