@@ -34,12 +34,34 @@
 
 namespace lbm_amd {
 
+// Element indices within a plane: 32-bit where a plane holds less than
+// 4 GiB (B32), so that an access is a uniform direction-row base plus one
+// 32-bit byte offset per lane; 64-bit otherwise.
+template <bool B32>
+struct AaIdx {
+    using type = int64_t;
+};
+template <>
+struct AaIdx<true> {
+    using type = uint32_t;
+};
+
+template <bool B32, class T, class I>
+__device__ __forceinline__ T& aa_ref(T* m, int q, int64_t plane, I e) {
+    if constexpr (B32)
+        return *reinterpret_cast<T*>(reinterpret_cast<char*>(m + q * plane) +
+                                     static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(T)));
+    else
+        return m[q * plane + e];
+}
+
 // Element of the neighbour x + sgn c_q (wrapped along periodic axes).
-template <class L, int q>
-__device__ __forceinline__ int64_t aa_nb(const FieldLayout& Ly, bool wraps, int64_t e, int x, int y, int z, int sgn) {
-    if (!wraps) return e + sgn * (L::cx(q) + L::cy(q) * Ly.px + L::cz(q) * Ly.pxy);
-    return cell_elem(Ly, wrap_if(x + sgn * L::cx(q), Ly.xl, Ly.per[0]), wrap_if(y + sgn * L::cy(q), Ly.yl, Ly.per[1]),
-                     wrap_if(z + sgn * L::cz(q), Ly.zl, Ly.per[2]));
+template <class L, int q, class I>
+__device__ __forceinline__ I aa_nb(const FieldLayout& Ly, bool wraps, I e, int x, int y, int z, int sgn) {
+    if (!wraps) return e + static_cast<I>(sgn * (L::cx(q) + L::cy(q) * Ly.px + L::cz(q) * Ly.pxy));
+    return static_cast<I>(Ly.off) + static_cast<I>(wrap_if(x + sgn * L::cx(q), Ly.xl, Ly.per[0])) +
+           static_cast<I>(Ly.px) * static_cast<I>(wrap_if(y + sgn * L::cy(q), Ly.yl, Ly.per[1])) +
+           static_cast<I>(Ly.pxy) * static_cast<I>(wrap_if(z + sgn * L::cz(q), Ly.zl, Ly.per[2]));
 }
 
 __device__ __forceinline__ bool aa_wraps(const FieldLayout& Ly, int x, int y, int z) {
@@ -94,7 +116,7 @@ __device__ __forceinline__ void aa_own(const typename Io<M, L>::Calc (&f)[L::Q],
     });
 }
 
-template <class L, Mode M>
+template <class L, Mode M, bool B32>
 __global__ void __launch_bounds__(256) k_aa_eo(StepArgs a, int tpp, double rxl) {
     using IO = Io<M, L>;
     using T = typename IO::T;
@@ -103,7 +125,8 @@ __global__ void __launch_bounds__(256) k_aa_eo(StepArgs a, int tpp, double rxl) 
     const FieldLayout& Ly = a.L;
     int x, y, z;
     if (!aa_tile_cell(Ly, tpp, rxl, x, y, z)) return;
-    const int64_t e = cell_elem(Ly, x, y, z);
+    using I = typename AaIdx<B32>::type;
+    const I e = static_cast<I>(cell_elem(Ly, x, y, z));
     const uint8_t k = a.kind[e];
     if (k >= KIND_COND0) return;
     T* __restrict__ m = static_cast<T*>(a.dst);
@@ -111,12 +134,12 @@ __global__ void __launch_bounds__(256) k_aa_eo(StepArgs a, int tpp, double rxl) 
     Calc f[Q];
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        f[q] = IO::template to_calc<q>(m[q * Ly.plane + aa_nb<L, q>(Ly, wraps, e, x, y, z, -1)]);
+        f[q] = IO::template to_calc<q>(aa_ref<B32>(m, q, Ly.plane, aa_nb<L, q>(Ly, wraps, e, x, y, z, -1)));
     });
-    aa_collide<L, M>(a, e, f);
+    aa_collide<L, M>(a, static_cast<int64_t>(e), f);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        m[(Q - 1 - q) * Ly.plane + aa_nb<L, q>(Ly, wraps, e, x, y, z, 1)] = IO::template from_calc<q>(f[q]);
+        aa_ref<B32>(m, Q - 1 - q, Ly.plane, aa_nb<L, q>(Ly, wraps, e, x, y, z, 1)) = IO::template from_calc<q>(f[q]);
     });
     if (k == KIND_NEAR) {
         Calc own[Q];
@@ -127,12 +150,13 @@ __global__ void __launch_bounds__(256) k_aa_eo(StepArgs a, int tpp, double rxl) 
             constexpr int j = Q - 1 - q;  // the link of x whose source is x + c_q
             const uint8_t kb = a.kind[aa_nb<L, q>(Ly, wraps, e, x, y, z, 1)];
             if (kb >= KIND_COND0 && kb < KIND_INERT)
-                m[q * Ly.plane + e] = IO::template from_calc<j>(rule_value<L, M, j>(a.conds[kb - KIND_COND0], os, own[q]));
+                aa_ref<B32>(m, q, Ly.plane, e) =
+                    IO::template from_calc<j>(rule_value<L, M, j>(a.conds[kb - KIND_COND0], os, own[q]));
         });
     }
 }
 
-template <class L, Mode M>
+template <class L, Mode M, bool B32>
 __global__ void __launch_bounds__(256) k_aa_oe(StepArgs a, int tpp, double rxl) {
     using IO = Io<M, L>;
     using T = typename IO::T;
@@ -141,19 +165,20 @@ __global__ void __launch_bounds__(256) k_aa_oe(StepArgs a, int tpp, double rxl) 
     const FieldLayout& Ly = a.L;
     int x, y, z;
     if (!aa_tile_cell(Ly, tpp, rxl, x, y, z)) return;
-    const int64_t e = cell_elem(Ly, x, y, z);
+    using I = typename AaIdx<B32>::type;
+    const I e = static_cast<I>(cell_elem(Ly, x, y, z));
     const uint8_t k = a.kind[e];
     if (k >= KIND_COND0) return;
     T* __restrict__ m = static_cast<T*>(a.dst);
     Calc f[Q];
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        f[q] = IO::template to_calc<q>(m[(Q - 1 - q) * Ly.plane + e]);
+        f[q] = IO::template to_calc<q>(aa_ref<B32>(m, Q - 1 - q, Ly.plane, e));
     });
-    aa_collide<L, M>(a, e, f);
+    aa_collide<L, M>(a, static_cast<int64_t>(e), f);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        m[q * Ly.plane + e] = IO::template from_calc<q>(f[q]);
+        aa_ref<B32>(m, q, Ly.plane, e) = IO::template from_calc<q>(f[q]);
     });
     if (k == KIND_NEAR) {
         const bool wraps = aa_wraps(Ly, x, y, z);
@@ -162,10 +187,10 @@ __global__ void __launch_bounds__(256) k_aa_oe(StepArgs a, int tpp, double rxl) 
         const OwnSums<Calc> os = own_sums<L, Calc>(own);
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const int64_t be = aa_nb<L, q>(Ly, wraps, e, x, y, z, -1);
+            const I be = aa_nb<L, q>(Ly, wraps, e, x, y, z, -1);
             const uint8_t kb = a.kind[be];
             if (kb >= KIND_COND0 && kb < KIND_INERT)
-                m[q * Ly.plane + be] =
+                aa_ref<B32>(m, q, Ly.plane, be) =
                     IO::template from_calc<q>(rule_value<L, M, q>(a.conds[kb - KIND_COND0], os, own[Q - 1 - q]));
         });
     }
@@ -174,24 +199,25 @@ __global__ void __launch_bounds__(256) k_aa_oe(StepArgs a, int tpp, double rxl) 
 // O -> E without a collision: swap M[q][x] <-> M[qbar][x + c_q]. A fluid cell
 // takes the pairs of its first Q/2 directions, and those of the others whose
 // partner cell is not fluid (that partner takes no pairs).
-template <class L, Mode M>
+template <class L, Mode M, bool B32>
 __global__ void __launch_bounds__(256) k_aa_swap(StepArgs a, int tpp, double rxl) {
     using T = typename Io<M, L>::T;
     constexpr int Q = L::Q;
     const FieldLayout& Ly = a.L;
     int x, y, z;
     if (!aa_tile_cell(Ly, tpp, rxl, x, y, z)) return;
-    const int64_t e = cell_elem(Ly, x, y, z);
+    using I = typename AaIdx<B32>::type;
+    const I e = static_cast<I>(cell_elem(Ly, x, y, z));
     if (a.kind[e] >= KIND_COND0) return;
     T* __restrict__ m = static_cast<T*>(a.dst);
     const bool wraps = aa_wraps(Ly, x, y, z);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (q != L::rest) {
-            const int64_t pe = aa_nb<L, q>(Ly, wraps, e, x, y, z, 1);
+            const I pe = aa_nb<L, q>(Ly, wraps, e, x, y, z, 1);
             if (q < Q / 2 || a.kind[pe] >= KIND_COND0) {
-                T* p0 = m + q * Ly.plane + e;
-                T* p1 = m + (Q - 1 - q) * Ly.plane + pe;
+                T* p0 = &aa_ref<B32>(m, q, Ly.plane, e);
+                T* p1 = &aa_ref<B32>(m, Q - 1 - q, Ly.plane, pe);
                 const T v0 = *p0;
                 *p0 = *p1;
                 *p1 = v0;

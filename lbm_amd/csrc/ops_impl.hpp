@@ -141,17 +141,29 @@ struct OpsImpl {
         return hipGetLastError();
     }
 
-    static hipError_t aa_step(const StepArgs& a, const StepPlan& p, int phase, hipStream_t s) {
+    // phase 0: E -> O step, 1: O -> E step, 2: O -> E swap without a collision
+    template <bool B32>
+    static void launch_aa(const StepArgs& a, const StepPlan& p, int phase, hipStream_t s) {
         const dim3 grid(static_cast<uint32_t>(p.nblocks_cells));
         if (phase == 0)
-            hipLaunchKernelGGL((k_aa_eo<L, M>), grid, dim3(256), 0, s, a, p.tpp, p.rxl);
+            hipLaunchKernelGGL((k_aa_eo<L, M, B32>), grid, dim3(256), 0, s, a, p.tpp, p.rxl);
+        else if (phase == 1)
+            hipLaunchKernelGGL((k_aa_oe<L, M, B32>), grid, dim3(256), 0, s, a, p.tpp, p.rxl);
         else
-            hipLaunchKernelGGL((k_aa_oe<L, M>), grid, dim3(256), 0, s, a, p.tpp, p.rxl);
+            hipLaunchKernelGGL((k_aa_swap<L, M, B32>), grid, dim3(256), 0, s, a, p.tpp, p.rxl);
+    }
+    static hipError_t aa_step(const StepArgs& a, const StepPlan& p, int phase, hipStream_t s) {
+        if (p.aa32)
+            launch_aa<true>(a, p, phase, s);
+        else
+            launch_aa<false>(a, p, phase, s);
         return hipGetLastError();
     }
     static hipError_t aa_swap(const StepArgs& a, const StepPlan& p, hipStream_t s) {
-        hipLaunchKernelGGL((k_aa_swap<L, M>), dim3(static_cast<uint32_t>(p.nblocks_cells)), dim3(256), 0, s, a,
-                           p.tpp, p.rxl);
+        if (p.aa32)
+            launch_aa<true>(a, p, 2, s);
+        else
+            launch_aa<false>(a, p, 2, s);
         return hipGetLastError();
     }
 
