@@ -57,6 +57,7 @@ struct BatchArgs {
     int parity;        // src = buf[parity], dst = buf[parity ^ 1]
     int materialised;  // the step pulls stored boundary values
     int freeslip;      // some domain has a FreeSlip condition (cell tiles built with its rule)
+    int idx32;         // every domain's plane is under 4 GiB (cell tiles use 32-bit byte offsets)
 };
 
 struct BatchOps {

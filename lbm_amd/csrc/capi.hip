@@ -123,7 +123,7 @@ int make_plan(lbm_domain* d) {
     p.rxl = 1.0 / static_cast<double>(d->L.xl);
     p.freeslip = false;
     for (const HostCond& c : d->conds) p.freeslip |= c.kind == LBM_BC_FREESLIP;
-    p.aa32 = d->L.plane * static_cast<int64_t>(d->esize) < (int64_t{1} << 32);
+    p.idx32 = d->L.plane * static_cast<int64_t>(d->esize) < (int64_t{1} << 32);
     p.cells = d->tune_vec == 0 && (d->nb > 0 || d->compact) &&
               p.nblocks_cells <= 0x7fffffffLL && (plane_cells + 63) / 64 <= 0x7fffffffLL;
     if (!p.cells && d->compact && V > 1)

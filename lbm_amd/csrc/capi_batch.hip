@@ -32,6 +32,7 @@ struct lbm_batch {
     int kernel = BK_CELLS;      // enum BatchKernel
     bool any_compact = false;   // some domain packs its rows: row kernels take V = 1 only
     bool any_freeslip = false;  // some domain has a FreeSlip condition
+    bool all_idx32 = true;      // every domain's plane is under 4 GiB
     int64_t near_cells = 0;     // NEAR cells of all domains
     BatchView* d_views = nullptr;
     TileRef* d_tiles = nullptr;
@@ -159,6 +160,7 @@ hipError_t enqueue(lbm_batch* b, uint64_t nsteps, int parity, int mat) {
     a.near = b->d_near;
     a.nnear = b->nnear;
     a.freeslip = b->any_freeslip ? 1 : 0;
+    a.idx32 = b->all_idx32 ? 1 : 0;
     // blocks that give every tile its own block (cell tiles: 4 wavefronts a block)
     const int64_t full = std::max<int64_t>(1, b->kernel == BK_CELLS ? (b->ntiles + 3) / 4 : b->ntiles);
     const int grid_bulk = static_cast<int>(b->bpc ? std::min<int64_t>(full, static_cast<int64_t>(b->ncu) * b->bpc)
@@ -217,6 +219,7 @@ int lbm_batch_create(lbm_domain* const* domains, int n, lbm_batch** out) {
         b->ncells += static_cast<uint64_t>(d->L.xl) * d->L.yl * d->L.zl;
         b->any_compact |= d->compact;
         b->any_freeslip |= d->plan.freeslip;
+        b->all_idx32 &= d->plan.idx32;
     }
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         delete b;

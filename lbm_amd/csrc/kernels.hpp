@@ -24,6 +24,8 @@
 //            one thread per cell, evaluates boundary rules.
 #pragma once
 
+#include <type_traits>
+
 #include "collide.hpp"
 #include "domain_view.hpp"
 #include "lbm_hip/lattice.hpp"
@@ -80,6 +82,29 @@ __device__ __forceinline__ void store_stream1(T* p, T v) {
 #else
     *p = v;
 #endif
+}
+
+// Element indices within a plane: 32-bit where a plane holds less than
+// 4 GiB (B32), so that an access is a uniform direction-plane base plus one
+// 32-bit byte offset per lane; 64-bit otherwise.
+template <bool B32>
+struct ElemIdx {
+    using type = int64_t;
+};
+template <>
+struct ElemIdx<true> {
+    using type = uint32_t;
+};
+
+// Element e of direction plane q of field m.
+template <bool B32, class T, class I>
+__device__ __forceinline__ T& plane_ref(T* m, int q, int64_t plane, I e) {
+    using Byte = std::conditional_t<std::is_const_v<T>, const char, char>;
+    if constexpr (B32)
+        return *reinterpret_cast<T*>(reinterpret_cast<Byte*>(m + q * plane) +
+                                     static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(T)));
+    else
+        return m[q * plane + e];
 }
 
 __device__ __forceinline__ int wrap1(int v, int l) { return v == 0 ? l : (v == l + 1 ? 1 : v); }
@@ -265,14 +290,14 @@ __device__ __forceinline__ OwnSums<Calc> own_sums(const Calc (&own)[L::Q]) {
 }
 
 // Own values read from the source field at element e (one pass, nothing kept).
-template <class L, Mode M>
+template <class L, Mode M, bool B32 = false>
 __device__ __forceinline__ OwnSums<typename Io<M, L>::Calc> own_sums_at(
-    const typename Io<M, L>::T* __restrict__ src, int64_t plane, int64_t e) {
+    const typename Io<M, L>::T* __restrict__ src, int64_t plane, typename ElemIdx<B32>::type e) {
     using IO = Io<M, L>;
     OwnAcc<L, typename IO::Calc> acc;
     static_for<0, L::Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        acc.template add<q>(IO::template to_calc<q>(src[q * plane + e]));
+        acc.template add<q>(IO::template to_calc<q>(plane_ref<B32>(src, q, plane, e)));
     });
     return acc.r;
 }
@@ -469,15 +494,18 @@ __device__ __forceinline__ void near_cell(const StepArgs& a, int x, int y, int z
 // along a walled axis the source may be a ghost or solid cell); a NEAR cell
 // then replaces the values pulled from non-fluid sources by their boundary
 // values (as near_cell), and all lanes collide and store together. The bits
-// equal those of bulk_cells / near_cell for the same cell.
-template <class L, Mode M, bool FS = true>
+// equal those of bulk_cells / near_cell for the same cell. B32 takes the
+// cell's own and plain-pull addresses as 32-bit byte offsets (planes under
+// 4 GiB).
+template <class L, Mode M, bool FS = true, bool B32 = false>
 __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
+    using I = typename ElemIdx<B32>::type;
     constexpr int Q = L::Q;
     const FieldLayout& Ly = a.L;
-    const int64_t e = cell_elem(Ly, x, y, z);
+    const I e = static_cast<I>(cell_elem(Ly, x, y, z));
     const uint8_t k = a.kind[e];
     const T* __restrict__ src = static_cast<const T*>(a.src);
     T* __restrict__ dst = static_cast<T*>(a.dst);
@@ -491,13 +519,12 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
     const bool wraps = (Ly.per[0] && (x == 1 || x == Ly.xl)) || (Ly.per[1] && (y == 1 || y == Ly.yl)) ||
                        (Ly.per[2] && (z == 1 || z == Ly.zl));
     if (!wraps) {
-        const T* __restrict__ at = src + e;
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
             const int64_t d = cx + (cy > 0 ? Ly.px : (cy < 0 ? -Ly.px : 0)) +
                               (cz > 0 ? Ly.pxy : (cz < 0 ? -Ly.pxy : 0));
-            f[q] = IO::template to_calc<q>(at[q * Ly.plane - d]);
+            f[q] = IO::template to_calc<q>(plane_ref<B32>(src, q, Ly.plane, e - static_cast<I>(d)));
         });
     } else {
         static_for<0, Q>([&](auto qc) {
@@ -506,15 +533,16 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
             const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
             const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
             const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
-            f[q] = IO::template to_calc<q>(src[q * Ly.plane + cell_elem(Ly, sx, sy, sz)]);
+            f[q] = IO::template to_calc<q>(
+                plane_ref<B32>(src, q, Ly.plane, static_cast<I>(cell_elem(Ly, sx, sy, sz))));
         });
     }
     if (k >= KIND_COND0) return;
     if (k == KIND_NEAR && !a.materialised) {
-        const OwnSums<Calc> os = own_sums_at<L, M>(src, Ly.plane, e);
+        const OwnSums<Calc> os = own_sums_at<L, M, B32>(src, Ly.plane, e);
         auto own = [&](auto kc) {
             constexpr int kk = decltype(kc)::value;
-            return IO::template to_calc<kk>(src[kk * Ly.plane + e]);
+            return IO::template to_calc<kk>(plane_ref<B32>(src, kk, Ly.plane, e));
         };
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
@@ -546,13 +574,13 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
     }
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        store_stream1<T>(dst + q * Ly.plane + e, IO::template from_calc<q>(f[q]));
+        store_stream1<T>(&plane_ref<B32>(dst, q, Ly.plane, e), IO::template from_calc<q>(f[q]));
     });
 }
 
 // Cell-tile kernel: wavefront w of the grid takes the 64 cells
 // 64 (w mod tpp) .. + 63 (x fastest) of the z plane 1 + w / tpp.
-template <class L, Mode M, bool FS = true>
+template <class L, Mode M, bool FS = true, bool B32 = false>
 __global__ void __launch_bounds__(256) k_cells(StepArgs a, int tpp, double rxl) {
     const int64_t w = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int z = 1 + static_cast<int>(w / tpp);
@@ -570,7 +598,7 @@ __global__ void __launch_bounds__(256) k_cells(StepArgs a, int tpp, double rxl) 
         ++y;
         x -= xl;
     }
-    any_cell<L, M, FS>(a, 1 + static_cast<int>(x), 1 + static_cast<int>(y), z);
+    any_cell<L, M, FS, B32>(a, 1 + static_cast<int>(x), 1 + static_cast<int>(y), z);
 }
 
 // Near-boundary kernel: one thread per listed NEAR cell.

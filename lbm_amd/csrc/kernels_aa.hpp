@@ -34,26 +34,6 @@
 
 namespace lbm_amd {
 
-// Element indices within a plane: 32-bit where a plane holds less than
-// 4 GiB (B32), so that an access is a uniform direction-row base plus one
-// 32-bit byte offset per lane; 64-bit otherwise.
-template <bool B32>
-struct AaIdx {
-    using type = int64_t;
-};
-template <>
-struct AaIdx<true> {
-    using type = uint32_t;
-};
-
-template <bool B32, class T, class I>
-__device__ __forceinline__ T& aa_ref(T* m, int q, int64_t plane, I e) {
-    if constexpr (B32)
-        return *reinterpret_cast<T*>(reinterpret_cast<char*>(m + q * plane) +
-                                     static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(T)));
-    else
-        return m[q * plane + e];
-}
 
 // Element of the neighbour x + sgn c_q (wrapped along periodic axes).
 template <class L, int q, class I>
@@ -125,7 +105,7 @@ __global__ void __launch_bounds__(256) k_aa_eo(StepArgs a, int tpp, double rxl) 
     const FieldLayout& Ly = a.L;
     int x, y, z;
     if (!aa_tile_cell(Ly, tpp, rxl, x, y, z)) return;
-    using I = typename AaIdx<B32>::type;
+    using I = typename ElemIdx<B32>::type;
     const I e = static_cast<I>(cell_elem(Ly, x, y, z));
     const uint8_t k = a.kind[e];
     if (k >= KIND_COND0) return;
@@ -134,12 +114,12 @@ __global__ void __launch_bounds__(256) k_aa_eo(StepArgs a, int tpp, double rxl) 
     Calc f[Q];
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        f[q] = IO::template to_calc<q>(aa_ref<B32>(m, q, Ly.plane, aa_nb<L, q>(Ly, wraps, e, x, y, z, -1)));
+        f[q] = IO::template to_calc<q>(plane_ref<B32>(m, q, Ly.plane, aa_nb<L, q>(Ly, wraps, e, x, y, z, -1)));
     });
     aa_collide<L, M>(a, static_cast<int64_t>(e), f);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        aa_ref<B32>(m, Q - 1 - q, Ly.plane, aa_nb<L, q>(Ly, wraps, e, x, y, z, 1)) = IO::template from_calc<q>(f[q]);
+        plane_ref<B32>(m, Q - 1 - q, Ly.plane, aa_nb<L, q>(Ly, wraps, e, x, y, z, 1)) = IO::template from_calc<q>(f[q]);
     });
     if (k == KIND_NEAR) {
         Calc own[Q];
@@ -150,7 +130,7 @@ __global__ void __launch_bounds__(256) k_aa_eo(StepArgs a, int tpp, double rxl) 
             constexpr int j = Q - 1 - q;  // the link of x whose source is x + c_q
             const uint8_t kb = a.kind[aa_nb<L, q>(Ly, wraps, e, x, y, z, 1)];
             if (kb >= KIND_COND0 && kb < KIND_INERT)
-                aa_ref<B32>(m, q, Ly.plane, e) =
+                plane_ref<B32>(m, q, Ly.plane, e) =
                     IO::template from_calc<j>(rule_value<L, M, j>(a.conds[kb - KIND_COND0], os, own[q]));
         });
     }
@@ -165,7 +145,7 @@ __global__ void __launch_bounds__(256) k_aa_oe(StepArgs a, int tpp, double rxl) 
     const FieldLayout& Ly = a.L;
     int x, y, z;
     if (!aa_tile_cell(Ly, tpp, rxl, x, y, z)) return;
-    using I = typename AaIdx<B32>::type;
+    using I = typename ElemIdx<B32>::type;
     const I e = static_cast<I>(cell_elem(Ly, x, y, z));
     const uint8_t k = a.kind[e];
     if (k >= KIND_COND0) return;
@@ -173,12 +153,12 @@ __global__ void __launch_bounds__(256) k_aa_oe(StepArgs a, int tpp, double rxl) 
     Calc f[Q];
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        f[q] = IO::template to_calc<q>(aa_ref<B32>(m, Q - 1 - q, Ly.plane, e));
+        f[q] = IO::template to_calc<q>(plane_ref<B32>(m, Q - 1 - q, Ly.plane, e));
     });
     aa_collide<L, M>(a, static_cast<int64_t>(e), f);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        aa_ref<B32>(m, q, Ly.plane, e) = IO::template from_calc<q>(f[q]);
+        plane_ref<B32>(m, q, Ly.plane, e) = IO::template from_calc<q>(f[q]);
     });
     if (k == KIND_NEAR) {
         const bool wraps = aa_wraps(Ly, x, y, z);
@@ -190,7 +170,7 @@ __global__ void __launch_bounds__(256) k_aa_oe(StepArgs a, int tpp, double rxl) 
             const I be = aa_nb<L, q>(Ly, wraps, e, x, y, z, -1);
             const uint8_t kb = a.kind[be];
             if (kb >= KIND_COND0 && kb < KIND_INERT)
-                aa_ref<B32>(m, q, Ly.plane, be) =
+                plane_ref<B32>(m, q, Ly.plane, be) =
                     IO::template from_calc<q>(rule_value<L, M, q>(a.conds[kb - KIND_COND0], os, own[Q - 1 - q]));
         });
     }
@@ -206,7 +186,7 @@ __global__ void __launch_bounds__(256) k_aa_swap(StepArgs a, int tpp, double rxl
     const FieldLayout& Ly = a.L;
     int x, y, z;
     if (!aa_tile_cell(Ly, tpp, rxl, x, y, z)) return;
-    using I = typename AaIdx<B32>::type;
+    using I = typename ElemIdx<B32>::type;
     const I e = static_cast<I>(cell_elem(Ly, x, y, z));
     if (a.kind[e] >= KIND_COND0) return;
     T* __restrict__ m = static_cast<T*>(a.dst);
@@ -216,8 +196,8 @@ __global__ void __launch_bounds__(256) k_aa_swap(StepArgs a, int tpp, double rxl
         if constexpr (q != L::rest) {
             const I pe = aa_nb<L, q>(Ly, wraps, e, x, y, z, 1);
             if (q < Q / 2 || a.kind[pe] >= KIND_COND0) {
-                T* p0 = &aa_ref<B32>(m, q, Ly.plane, e);
-                T* p1 = &aa_ref<B32>(m, Q - 1 - q, Ly.plane, pe);
+                T* p0 = &plane_ref<B32>(m, q, Ly.plane, e);
+                T* p1 = &plane_ref<B32>(m, Q - 1 - q, Ly.plane, pe);
                 const T v0 = *p0;
                 *p0 = *p1;
                 *p1 = v0;

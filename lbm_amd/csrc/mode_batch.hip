@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256) k_batch_bulk(BatchArgs b) {
 // lane takes one cell of its tile.
 // The tile index is wavefront-uniform; readfirstlane states it, so the tile
 // and the domain view are read into scalar registers once per wavefront.
-template <class L, Mode M, bool FS>
+template <class L, Mode M, bool FS, bool B32>
 __global__ void __launch_bounds__(256) k_batch_cells(BatchArgs b) {
     const int lane = threadIdx.x & 63;
     const int nwaves = static_cast<int>(gridDim.x) * (static_cast<int>(blockDim.x) >> 6);
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256) k_batch_cells(BatchArgs b) {
                 x -= v.L.xl;
             }
             const StepArgs a = view_args(v, b.parity, b.materialised);
-            any_cell<L, M, FS>(a, 1 + x, 1 + y, z);
+            any_cell<L, M, FS, B32>(a, 1 + x, 1 + y, z);
         }
     }
 }
@@ -108,10 +108,17 @@ struct BatchImpl {
                              hipStream_t s) {
         if (kernel == BK_CELLS) {
             if (a.ntiles > 0) {
-                if (a.freeslip)
-                    hipLaunchKernelGGL((k_batch_cells<L, M, true>), dim3(grid_bulk), dim3(256), 0, s, a);
-                else
-                    hipLaunchKernelGGL((k_batch_cells<L, M, false>), dim3(grid_bulk), dim3(256), 0, s, a);
+                if (a.freeslip) {
+                    if (a.idx32)
+                        hipLaunchKernelGGL((k_batch_cells<L, M, true, true>), dim3(grid_bulk), dim3(256), 0, s, a);
+                    else
+                        hipLaunchKernelGGL((k_batch_cells<L, M, true, false>), dim3(grid_bulk), dim3(256), 0, s, a);
+                } else {
+                    if (a.idx32)
+                        hipLaunchKernelGGL((k_batch_cells<L, M, false, true>), dim3(grid_bulk), dim3(256), 0, s, a);
+                    else
+                        hipLaunchKernelGGL((k_batch_cells<L, M, false, false>), dim3(grid_bulk), dim3(256), 0, s, a);
+                }
             }
             return hipGetLastError();
         }

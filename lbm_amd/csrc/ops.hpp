@@ -46,7 +46,7 @@ struct StepPlan {
     // Cell tiles (k_cells) instead of k_bulk + k_near: 64 consecutive cells of
     // a z plane per wavefront, every cell kind in one pass.
     bool cells;
-    bool aa32;             // in-place kernels index a plane with 32-bit offsets (plane < 4 GiB)
+    bool idx32;            // cell-tile and in-place kernels index a plane with 32-bit byte offsets (plane < 4 GiB)
     bool freeslip;         // a FreeSlip condition exists (cell tiles built with its rule)
     int tpp;               // 64-cell tiles per z plane
     int64_t nblocks_cells; // blocks of 4 wavefronts

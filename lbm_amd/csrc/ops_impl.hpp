@@ -36,15 +36,26 @@ struct OpsImpl {
         return true;
     }
 
+    template <bool FS, bool B32>
+    static void launch_cells(const StepArgs& a, const StepPlan& p, hipStream_t s) {
+        hipLaunchKernelGGL((k_cells<L, M, FS, B32>), dim3(static_cast<uint32_t>(p.nblocks_cells)), dim3(256), 0, s,
+                           a, p.tpp, p.rxl);
+    }
+
     template <int... Vs>
     static hipError_t step_impl(const StepArgs& a, const StepPlan& p, hipStream_t s, EventLog* log) {
         if (p.cells) {
-            if (p.freeslip)
-                hipLaunchKernelGGL((k_cells<L, M, true>), dim3(static_cast<uint32_t>(p.nblocks_cells)), dim3(256), 0,
-                                   s, a, p.tpp, p.rxl);
-            else
-                hipLaunchKernelGGL((k_cells<L, M, false>), dim3(static_cast<uint32_t>(p.nblocks_cells)), dim3(256), 0,
-                                   s, a, p.tpp, p.rxl);
+            if (p.freeslip) {
+                if (p.idx32)
+                    launch_cells<true, true>(a, p, s);
+                else
+                    launch_cells<true, false>(a, p, s);
+            } else {
+                if (p.idx32)
+                    launch_cells<false, true>(a, p, s);
+                else
+                    launch_cells<false, false>(a, p, s);
+            }
             if (log) log->mark(s, 0);
             return hipGetLastError();
         }
@@ -153,14 +164,14 @@ struct OpsImpl {
             hipLaunchKernelGGL((k_aa_swap<L, M, B32>), grid, dim3(256), 0, s, a, p.tpp, p.rxl);
     }
     static hipError_t aa_step(const StepArgs& a, const StepPlan& p, int phase, hipStream_t s) {
-        if (p.aa32)
+        if (p.idx32)
             launch_aa<true>(a, p, phase, s);
         else
             launch_aa<false>(a, p, phase, s);
         return hipGetLastError();
     }
     static hipError_t aa_swap(const StepArgs& a, const StepPlan& p, hipStream_t s) {
-        if (p.aa32)
+        if (p.idx32)
             launch_aa<true>(a, p, 2, s);
         else
             launch_aa<false>(a, p, 2, s);
