@@ -107,6 +107,19 @@ __device__ __forceinline__ T& plane_ref(T* m, int q, int64_t plane, I e) {
         return m[q * plane + e];
 }
 
+// cell_elem in the index type of plane_ref. With B32 every element index of
+// the field is below 2^32 and all terms are non-negative, so 32-bit
+// arithmetic gives it exactly.
+template <bool B32>
+__device__ __forceinline__ typename ElemIdx<B32>::type elem_at(const FieldLayout& L, int x, int y, int z) {
+    if constexpr (B32)
+        return static_cast<uint32_t>(L.off) + static_cast<uint32_t>(x) +
+               static_cast<uint32_t>(L.px) * static_cast<uint32_t>(y) +
+               static_cast<uint32_t>(L.pxy) * static_cast<uint32_t>(z);
+    else
+        return cell_elem(L, x, y, z);
+}
+
 __device__ __forceinline__ int wrap1(int v, int l) { return v == 0 ? l : (v == l + 1 ? 1 : v); }
 
 __device__ __forceinline__ int wrap_if(int v, int l, int per) {
@@ -371,11 +384,12 @@ constexpr int dir_of(int cx, int cy, int cz) {
 // Sets stale when the rule writes nothing (FreeSlip with no fluid candidate);
 // the value returned is then b's stored value, which is what the reference
 // reads too. FS = false builds the rule without the FreeSlip case, for
-// domains that have no FreeSlip condition.
-template <class L, Mode M, int q, bool FS = true, class OwnFn>
+// domains that have no FreeSlip condition. B32 indexes with 32-bit offsets
+// (planes under 4 GiB).
+template <class L, Mode M, int q, bool FS = true, bool B32 = false, class OwnFn>
 __device__ __forceinline__ typename Io<M, L>::Calc pull_boundary(
     const StepArgs& a, const Cond& cd, const OwnSums<typename Io<M, L>::Calc>& os, OwnFn own, int x,
-    int y, int z, int sx, int sy, int sz, int64_t se, bool& stale) {
+    int y, int z, int sx, int sy, int sz, typename ElemIdx<B32>::type se, bool& stale) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     constexpr int Q = L::Q;
@@ -396,22 +410,22 @@ __device__ __forceinline__ typename Io<M, L>::Calc pull_boundary(
                                              kFsSwap[j][2] ? -cz : cz);
                 static_assert(rq >= 0, "reflected velocity outside the lattice");
                 if (!found) {
-                    const int64_t ce = cell_elem(Ly, kFsSwap[j][0] ? x : sx, kFsSwap[j][1] ? y : sy,
+                    const auto ce = elem_at<B32>(Ly, kFsSwap[j][0] ? x : sx, kFsSwap[j][1] ? y : sy,
                                                  kFsSwap[j][2] ? z : sz);
                     if (a.kind[ce] < KIND_COND0) {
-                        v = IO::template to_calc<rq>(src[rq * Ly.plane + ce]);
+                        v = IO::template to_calc<rq>(plane_ref<B32>(src, rq, Ly.plane, ce));
                         found = true;
                     }
                 }
             });
             if (found) return v;
             stale = true;
-            return IO::template to_calc<q>(src[q * Ly.plane + se]);
+            return IO::template to_calc<q>(plane_ref<B32>(src, q, Ly.plane, se));
         }
     }
     if (cd.kind < BC_NOSLIP || cd.kind > BC_PRESSURE) {
         stale = true;
-        return IO::template to_calc<q>(src[q * Ly.plane + se]);
+        return IO::template to_calc<q>(plane_ref<B32>(src, q, Ly.plane, se));
     }
     return rule_value<L, M, q>(cd, os, own(std::integral_constant<int, Q - 1 - q>{}));
 }
@@ -505,7 +519,7 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
     using I = typename ElemIdx<B32>::type;
     constexpr int Q = L::Q;
     const FieldLayout& Ly = a.L;
-    const I e = static_cast<I>(cell_elem(Ly, x, y, z));
+    const I e = elem_at<B32>(Ly, x, y, z);
     const uint8_t k = a.kind[e];
     const T* __restrict__ src = static_cast<const T*>(a.src);
     T* __restrict__ dst = static_cast<T*>(a.dst);
@@ -533,8 +547,7 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
             const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
             const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
             const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
-            f[q] = IO::template to_calc<q>(
-                plane_ref<B32>(src, q, Ly.plane, static_cast<I>(cell_elem(Ly, sx, sy, sz))));
+            f[q] = IO::template to_calc<q>(plane_ref<B32>(src, q, Ly.plane, elem_at<B32>(Ly, sx, sy, sz)));
         });
     }
     if (k >= KIND_COND0) return;
@@ -550,12 +563,12 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
             const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
             const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
             const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
-            const int64_t se = cell_elem(Ly, sx, sy, sz);
+            const I se = elem_at<B32>(Ly, sx, sy, sz);
             const uint8_t sk = a.kind[se];
             if (sk >= KIND_COND0) {
                 bool stale;
-                f[q] = pull_boundary<L, M, q, FS>(a, a.conds[sk - KIND_COND0], os, own, x, y, z, sx, sy, sz, se,
-                                              stale);
+                f[q] = pull_boundary<L, M, q, FS, B32>(a, a.conds[sk - KIND_COND0], os, own, x, y, z, sx, sy,
+                                                       sz, se, stale);
             }
         });
     }
