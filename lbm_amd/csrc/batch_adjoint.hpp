@@ -39,6 +39,7 @@ struct BatchAdjArgs {
     int gprev_slot, mu_in_slot, mu_out_slot; // adjoint step
     int terminal;
     int materialised;
+    int idx32;  // every domain's plane is under 4 GiB (32-bit byte offsets)
 };
 
 struct BatchAdjOps {

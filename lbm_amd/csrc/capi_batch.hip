@@ -656,6 +656,7 @@ int lbm_batch_adj_backward(lbm_batch_tape* t, double* grad_wall) {
     a.views = t->d_views;
     a.tiles = t->d_tiles;
     a.ntiles = t->ntiles;
+    a.idx32 = b->all_idx32 ? 1 : 0;
     const int grid = grid_for(t);
     Events ev(3 * t->nck);
     HIPCHK(ev.create());

@@ -328,13 +328,16 @@ __device__ __forceinline__ void adj_cell(const AdjArgs& a, int x, int y, int z, 
 // reverse-pull lambda plainly, collide and store together; a NEAR lane then
 // replaces the values on its boundary links and adds the wall terms, with the
 // operations of adj_cell in the same order, so mu and the gradients carry
-// the same bits as k_adj_bulk / k_adj_near.
-template <class L, Mode M, bool BRINK>
-__device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int z, int64_t e, bool near,
-                                             double* __restrict__ gw) {
+// the same bits as k_adj_bulk / k_adj_near. B32 indexes with 32-bit byte
+// offsets (planes of the field and of mu under 4 GiB).
+template <class L, Mode M, bool BRINK, bool B32 = false>
+__device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int z, typename ElemIdx<B32>::type e,
+                                             bool near, double* __restrict__ gw) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
+    using I = typename ElemIdx<B32>::type;
+    static_assert(!B32 || sizeof(T) == sizeof(Calc), "mu planes must be no larger than field planes");
     constexpr int Q = L::Q;
     const FieldLayout& Ly = a.L;
     const T* __restrict__ gp = static_cast<const T*>(a.gprev);
@@ -342,18 +345,18 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
     const bool wraps = (Ly.per[0] && (x == 1 || x == Ly.xl)) || (Ly.per[1] && (y == 1 || y == Ly.yl)) ||
                        (Ly.per[2] && (z == 1 || z == Ly.zl));
     // element offset of x + s c_q (s = -1: source, +1: consumer)
-    auto nb_elem = [&](auto qc, int sgn) -> int64_t {
+    auto nb_elem = [&](auto qc, int sgn) -> I {
         constexpr int q = decltype(qc)::value;
         if (!wraps)
-            return e + sgn * (L::cx(q) + L::cy(q) * Ly.px + L::cz(q) * Ly.pxy);
-        return cell_elem(Ly, wrap_if(x + sgn * L::cx(q), Ly.xl, Ly.per[0]),
-                         wrap_if(y + sgn * L::cy(q), Ly.yl, Ly.per[1]), wrap_if(z + sgn * L::cz(q), Ly.zl, Ly.per[2]));
+            return e + static_cast<I>(sgn * (L::cx(q) + L::cy(q) * Ly.px + L::cz(q) * Ly.pxy));
+        return elem_at<B32>(Ly, wrap_if(x + sgn * L::cx(q), Ly.xl, Ly.per[0]),
+                            wrap_if(y + sgn * L::cy(q), Ly.yl, Ly.per[1]), wrap_if(z + sgn * L::cz(q), Ly.zl, Ly.per[2]));
     };
 
     Calc h[Q];
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        h[q] = IO::template to_calc<q>(gp[q * Ly.plane + nb_elem(qc, -1)]);
+        h[q] = IO::template to_calc<q>(plane_ref<B32>(gp, q, Ly.plane, nb_elem(qc, -1)));
     });
     if (near && !a.materialised) {
         StepArgs sa{};
@@ -361,22 +364,22 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
         sa.src = a.gprev;
         sa.kind = a.kind;
         sa.conds = a.conds;
-        const OwnSums<Calc> os = own_sums_at<L, M>(gp, Ly.plane, e);
+        const OwnSums<Calc> os = own_sums_at<L, M, B32>(gp, Ly.plane, e);
         auto own = [&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            return IO::template to_calc<k>(gp[k * Ly.plane + e]);
+            return IO::template to_calc<k>(plane_ref<B32>(gp, k, Ly.plane, e));
         };
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             const int sx = wrap_if(x - L::cx(q), Ly.xl, Ly.per[0]);
             const int sy = wrap_if(y - L::cy(q), Ly.yl, Ly.per[1]);
             const int sz = wrap_if(z - L::cz(q), Ly.zl, Ly.per[2]);
-            const int64_t se = cell_elem(Ly, sx, sy, sz);
+            const I se = elem_at<B32>(Ly, sx, sy, sz);
             const uint8_t k = a.kind[se];
             if (k >= KIND_COND0) {
                 bool stale;
-                h[q] = pull_boundary<L, M, q, false>(sa, a.conds[k - KIND_COND0], os, own, x, y, z, sx, sy, sz, se,
-                                                     stale);
+                h[q] = pull_boundary<L, M, q, false, B32>(sa, a.conds[k - KIND_COND0], os, own, x, y, z, sx, sy, sz,
+                                                          se, stale);
             }
         });
     }
@@ -387,13 +390,13 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
         Calc gf[Q];
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            gf[q] = IO::template to_calc<q>(gc[q * Ly.plane + e]);
+            gf[q] = IO::template to_calc<q>(plane_ref<B32>(gc, q, Ly.plane, e));
         });
         objective_grad<L, Calc>(a, x, y, z, gf, lam);
     } else {
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            lam[q] = mi[q * Ly.plane + nb_elem(qc, 1)];
+            lam[q] = plane_ref<B32>(mi, q, Ly.plane, nb_elem(qc, 1));
         });
         if (near) {
             Calc wall_term = Calc(0);
@@ -402,14 +405,15 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
                 const uint8_t k = a.kind[nb_elem(qc, 1)];
                 if (k >= KIND_COND0) {
                     const int kind = a.conds[k - KIND_COND0].kind;
-                    lam[q] = (kind == BC_NOSLIP || kind == BC_MOVINGWALL) ? mi[(Q - 1 - q) * Ly.plane + e] : Calc(0);
+                    lam[q] = (kind == BC_NOSLIP || kind == BC_MOVINGWALL) ? plane_ref<B32>(mi, Q - 1 - q, Ly.plane, e)
+                                                                          : Calc(0);
                 }
                 const uint8_t ks = a.kind[nb_elem(qc, -1)];
                 if (ks >= KIND_COND0 && ks < KIND_INERT) {
                     const Cond& cd = a.conds[ks - KIND_COND0];
                     if (cd.kind == BC_MOVINGWALL) {
                         const double cu = L::cx(q) * cd.vel[0] + L::cy(q) * cd.vel[1] + L::cz(q) * cd.vel[2];
-                        wall_term += mi[q * Ly.plane + e] * static_cast<Calc>(6.0 * weight<L>(q) * cu);
+                        wall_term += plane_ref<B32>(mi, q, Ly.plane, e) * static_cast<Calc>(6.0 * weight<L>(q) * cu);
                     }
                 }
             });
@@ -426,7 +430,7 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
     Calc* __restrict__ mo = static_cast<Calc*>(a.mu_out);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        mo[q * Ly.plane + e] = mu[q];
+        plane_ref<B32>(mo, q, Ly.plane, e) = mu[q];
     });
     if constexpr (BRINK) a.grad_alpha[e] += static_cast<double>(da);
 
@@ -434,7 +438,7 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
         Calc rho_own = Calc(1);
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            rho_own += IO::template to_calc<q>(gp[q * Ly.plane + e]);
+            rho_own += IO::template to_calc<q>(plane_ref<B32>(gp, q, Ly.plane, e));
         });
         bool walls = false;
         double gw0 = 0, gw1 = 0, gw2 = 0;

@@ -40,7 +40,7 @@ __device__ __forceinline__ T* slot(const BatchAdjView& v, int i) {
 }
 
 // One forward step of every domain from slot src_slot to slot dst_slot.
-template <class L, Mode M>
+template <class L, Mode M, bool B32>
 __global__ void __launch_bounds__(256) k_badj_recompute(BatchAdjArgs b) {
     using T = typename StorageOf<M>::T;
     const int lane = threadIdx.x & 63;
@@ -61,13 +61,13 @@ __global__ void __launch_bounds__(256) k_badj_recompute(BatchAdjArgs b) {
         a.materialised = b.materialised;
         a.tau = v.tau;
         a.omega = v.omega;
-        any_cell<L, M, false>(a, x, y, z);  // the adjoint takes no FreeSlip condition
+        any_cell<L, M, false, B32>(a, x, y, z);  // the adjoint takes no FreeSlip condition
     }
 }
 
 // One adjoint step of every domain: g^{t-1} in gprev_slot, mu^{t+1} in
 // mu_in_slot, mu^t into mu_out_slot; kinetic-energy objective.
-template <class L, Mode M>
+template <class L, Mode M, bool B32>
 __global__ void __launch_bounds__(256) k_badj_backward(BatchAdjArgs b) {
     using T = typename StorageOf<M>::T;
     static_assert(sizeof(T) == sizeof(typename StorageOf<M>::Calc), "mu slots hold the arithmetic type");
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(256) k_badj_backward(BatchAdjArgs b) {
         const BatchAdjView& v = b.views[tr.dom];
         int x, y, z;
         if (!tile_cell(v, tr.local, lane, x, y, z)) continue;
-        const int64_t e = cell_elem(v.L, x, y, z);
+        const auto e = elem_at<B32>(v.L, x, y, z);
         const uint8_t k = v.kind[e];
         if (k >= KIND_COND0) continue;
         AdjArgs a{};
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) k_badj_backward(BatchAdjArgs b) {
         a.tau = v.tau;
         a.omega = v.omega;
         a.objective = OBJ_KINETIC;
-        adj_any_cell<L, M, false>(a, x, y, z, e, k == KIND_NEAR, v.gw + 3 * e);
+        adj_any_cell<L, M, false, B32>(a, x, y, z, e, k == KIND_NEAR, v.gw + 3 * static_cast<size_t>(e));
     }
 }
 
@@ -132,11 +132,17 @@ __global__ void __launch_bounds__(256) k_badj_reduce(const BatchAdjView* views, 
 template <class L, Mode M>
 struct BatchAdjImpl {
     static hipError_t recompute(const BatchAdjArgs& a, int grid, hipStream_t s) {
-        hipLaunchKernelGGL((k_badj_recompute<L, M>), dim3(grid), dim3(256), 0, s, a);
+        if (a.idx32)
+            hipLaunchKernelGGL((k_badj_recompute<L, M, true>), dim3(grid), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_badj_recompute<L, M, false>), dim3(grid), dim3(256), 0, s, a);
         return hipGetLastError();
     }
     static hipError_t backward(const BatchAdjArgs& a, int grid, hipStream_t s) {
-        hipLaunchKernelGGL((k_badj_backward<L, M>), dim3(grid), dim3(256), 0, s, a);
+        if (a.idx32)
+            hipLaunchKernelGGL((k_badj_backward<L, M, true>), dim3(grid), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_badj_backward<L, M, false>), dim3(grid), dim3(256), 0, s, a);
         return hipGetLastError();
     }
     static hipError_t reduce(const BatchAdjView* views, int ndom, double* out, hipStream_t s) {
