@@ -65,20 +65,21 @@ def tgv(n: int, seed: int = 12345):
     return rho, u
 
 
-def traffic_from_profiles(mode: str, kernel: str = "bulk"):
-    """HBM bytes per launch of the kernel (bulk: k_bulk, pass: k_bulk2)
-    measured by rocprofv3 --pmc for this workload (FETCH_SIZE + WRITE_SIZE,
-    each scaled by the copy-kernel calibration), from the committed summary;
-    None when absent."""
-    p = ROOT / "profiles" / f"pmc_{kernel}_d3q19_256_{mode}.txt"
+def traffic_from_profiles(mode: str, kernel: str = "bulk", workload: str = "d3q19_256"):
+    """HBM bytes per launch of a kernel (bulk: k_bulk, pass: k_bulk2, cells:
+    k_cells) on a workload, measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, each scaled by the copy-kernel calibration), from the
+    committed summary profiles/pmc_<kernel>_<workload>_<mode>.txt; None when
+    absent."""
+    p = ROOT / "profiles" / f"pmc_{kernel}_{workload}_{mode}.txt"
     if not p.exists():
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text(), flags=re.M)
     return float(m.group(1)) if m else None
 
 
-def run_mode(capi, mode: str, steps: int, warmup: int, rho, u, steps_per_pass: int = 2):
-    d = capi.Domain(N, N, N, TAU, q=Q, storage=mode, periodic=(True, True, True))
+def run_mode(capi, mode: str, steps: int, warmup: int, rho, u, steps_per_pass: int = 2, q: int = Q):
+    d = capi.Domain(N, N, N, TAU, q=q, storage=mode, periodic=(True, True, True))
     d.finalize()
     d.set_time_blocking(steps_per_pass)
     d.init_equilibrium(rho, u)
@@ -96,21 +97,116 @@ def run_mode(capi, mode: str, steps: int, warmup: int, rho, u, steps_per_pass: i
     # launches in the timed region (an odd count adds one single step)
     launches = steps // spp + steps % spp
     kernel_ms = ev["bulk_ms"] / launches
-    alg_bytes = 2 * Q * BYTES[mode] * cells * spp
+    # bytes one launch must move: every population read once and written
+    # once, whether the launch advances one step or two
+    moved = 2 * q * BYTES[mode] * cells
     return {
         "kernel": "k_bulk2" if spp == 2 else "k_bulk",
         "steps_per_pass": spp,
         "z_chunk": info["z_chunk"] if spp == 2 else None,
         "mlups": cells * steps / wall / 1e6,
+        "mlups_ghost_inclusive": (N + 2) ** 3 * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "event_ms_per_step": ev["total_ms"] / steps,
         "kernel_ms": kernel_ms,
-        "alg_bytes": alg_bytes,
-        "achieved_gbs": alg_bytes / (kernel_ms * 1e-3) / 1e9,
+        "bytes_per_launch": moved,
+        "achieved_gbs": moved / (kernel_ms * 1e-3) / 1e9,
+        # step-equivalent rate: 2*Q*s bytes per cell-update over the launch
+        # time; above the HBM peak where a launch advances two steps
+        "step_equiv_gbs": moved * spp / (kernel_ms * 1e-3) / 1e9,
         "mass": float(mass[0]),
         "vec": info["vec"],
         "block": [info["block_x"], info["block_y"]],
     }
+
+
+def mode_row(r: dict, mode: str, q: int = Q) -> dict:
+    return {"mlups": round(r["mlups"], 1), "kernel": r["kernel"], "steps_per_launch": r["steps_per_pass"],
+            "kernel_ms": round(r["kernel_ms"], 5), "bytes_per_launch": r["bytes_per_launch"],
+            "achieved_gbs": round(r["achieved_gbs"], 1),
+            "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "step_equiv_gbs": round(r["step_equiv_gbs"], 1),
+            "bytes_per_cell_step": 2 * q * BYTES[mode]}
+
+
+def walled_case(capi, name: str, q: int, mode: str, dims, tau: float, sc, steps: int, warmup: int = 5):
+    """One walled workload, as two steps per pass (k_wall2, forced where the
+    domain allows it) and as single steps (64-cell tiles, k_cells), with the
+    form lbm_step picks by default named: event-timed kernel time per
+    launch, MLUPS over interior and
+    over fluid cells, and the roofline on the bytes a launch must move (each
+    population of every fluid cell read once and written once: 2*Q*s per
+    fluid cell, for one step or two; solid cells are not updated)."""
+    out = {}
+    for spp in (2, 1):
+        d = capi.Domain(*dims, tau, q=q, storage=mode)
+        sc.apply(d)
+        d.finalize()
+        if spp == 2:
+            out["default_steps_per_pass"] = d.info()["steps_per_pass"]
+        d.set_time_blocking(spp)
+        info = d.info()
+        d.step(warmup)
+        d.sync()
+        ev = d.time_steps(steps)
+        d.sync()
+        d.close()
+        k = info["steps_per_pass"]
+        launches = steps // k + steps % k
+        kernel_ms = ev["bulk_ms"] / launches
+        interior, fluid = info["ncells_interior"], info["ncells_fluid"]
+        moved = 2 * q * BYTES[mode] * fluid
+        gbs = moved / (kernel_ms * 1e-3) / 1e9
+        kname = "k_wall2" if k == 2 else ("k_cells" if info["cell_tiles"] else "k_bulk+k_near")
+        traffic = traffic_from_profiles(mode, "wall2" if k == 2 else "cells", name)
+        row = {"kernel": kname, "steps_per_launch": k, "kernel_ms": round(kernel_ms, 5),
+               "ms_per_step": round(ev["total_ms"] / steps, 5),
+               "mlups": round(interior / (ev["total_ms"] / steps) / 1e3, 1),
+               "mlups_fluid": round(fluid / (ev["total_ms"] / steps) / 1e3, 1),
+               "bytes_per_launch": moved, "achieved_gbs": round(gbs, 1),
+               "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+               "traffic_over_bytes": round(traffic / moved, 4) if traffic else None}
+        if k == 2 or spp == 1:
+            out["pass" if k == 2 else "single_step"] = row
+        if spp == 2 and k == 1:
+            out["pass"] = None
+        if spp == 1:
+            out.update({"lattice": f"D3Q{q}", "storage": mode, "shape": list(dims), "tau": tau, "steps": steps,
+                        "interior_cells": interior, "fluid_cells": fluid, "near_cells": info["ncells_near"]})
+    head = out["pass"] if out["default_steps_per_pass"] == 2 else out["single_step"]
+    out["mlups"] = head["mlups"]
+    return out
+
+
+def walled_leg(capi, steps: int, modes=("f32", "f16")):
+    """The scenarios the reference runs are walled: a 256^3 lid-driven cavity
+    (D3Q19, tau 0.6, lid 0.05 along x on zmax, the reference's cavity.xml
+    geometry) and the 384x192x192 D3Q27 obstacle channel (inflow 0.04 at x0,
+    outflow at xmax, no-slip sides, seeded spheres to a solid fraction of 0.2,
+    tau 0.55)."""
+    from lbm_amd import scenario, workloads
+    out = {}
+    cav = scenario.cavity(N, N, N, lid=(0.05, 0.0, 0.0))
+    for m in modes:
+        out[f"cavity256_q19_{m}"] = walled_case(capi, "cavity256_q19", 19, m, (N, N, N), 0.6, cav, steps)
+    ch = workloads.sphere_channel(384, 192, 192, u_in=0.04, solid_fraction=0.2, seed=12345)
+    for m in modes:
+        out[f"channel384_q27_{m}"] = walled_case(capi, "channel384_q27", 27, m, (384, 192, 192), 0.55, ch,
+                                                 max(steps // 2, 1))
+    return out
+
+
+def report_leg(capi, steps: int, warmup: int, rho, u):
+    """The lattice and precision rows of the measurement table on the same
+    periodic 256^3 TGV: D3Q19 fp64 (the reference's operation order), D3Q15
+    fp32/fp64, D3Q27 fp32/fp16/bf16/fp64, each in lbm_step's default form."""
+    rows = {}
+    for q, mode in ((19, "f64"), (15, "f32"), (15, "f64"), (27, "f32"), (27, "f16"), (27, "bf16"), (27, "f64")):
+        r = run_mode(capi, mode, steps, warmup, rho, u, q=q)
+        row = mode_row(r, mode, q)
+        row["mlups_ghost_inclusive"] = round(r["mlups_ghost_inclusive"], 1)
+        rows[f"d3q{q}_{mode}"] = row
+    return rows
 
 
 def aa_leg(capi, mode: str, steps: int, rho, u):
@@ -134,9 +230,22 @@ def aa_leg(capi, mode: str, steps: int, rho, u):
             "two_buffer_device_gb": out["ab"]["device_gb"]}
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(rho, u, steps: int = 4):
     """The fp64 CPU oracle (the reference algorithm in its AoS layout) on the
-    same 256^3 D3Q19 periodic workload, OpenMP over the host cores."""
+    same 256^3 D3Q19 periodic workload, OpenMP over the host cores. MLUPS are
+    given over interior cells and in the reference's own ghost-inclusive
+    form (2+xl)(2+yl)(2+zl) * steps / t (src/main.cpp:64-66)."""
     from oracle import ora
     o = ora.OracleDomain(N, N, N, TAU, q=Q, periodic=(True, True, True))
     o.finalize()
@@ -148,9 +257,15 @@ def cpu_baseline(rho, u, steps: int = 4):
     o.close()
     cores = ora.num_threads()
     return {"value": N ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
+            "mlups_ghost_inclusive": (N + 2) ** 3 * steps / dt / 1e6,
+            "cpu_model": cpu_model(),
             "sample": f"fp64 CPU oracle, D3Q19 256^3 periodic TGV, {steps} steps after 1 untimed, "
                       f"{cores} OpenMP threads ({os.cpu_count()} logical CPUs visible), "
-                      f"{dt:.1f} s"}
+                      f"{dt:.1f} s",
+            "layout": "the reference's AoS fp64 cells (Q values per cell) in two ghosted fields, pull "
+                      "stream, then the two-pass collide; boundary handlers are a uint16 id per cell "
+                      "dispatched by a switch, where the reference holds a Collision pointer per "
+                      "cell and calls through a vtable"}
 
 
 def adjoint_leg(capi, steps: int = 2000):
@@ -200,9 +315,21 @@ def batch_leg(capi, steps: int = 2000, n: int = 512, adj_steps: int = 100):
     b.step(20)
     b.sync()
     t = b.time_steps(steps)
+    # tau = 3 U L / Re + 1/2 reaches 0.5024 for small, high-Re cavities, where
+    # BGK is unstable: the device-side flags name the cavities that diverged
+    bad = b.status()
+    cells = np.array([d.xl * d.yl * d.zl for d in doms])
+    finite_cells = int(cells[~bad].sum())
+    ms = t["ms"] / steps
     out = {"workload": f"{n} D3Q19 lid-driven cavities, edges log-uniform [16,96], Re U[10,1000], fp32",
-           "steps": steps, "cells": t["cells"], "ms_per_step": round(t["ms"] / steps, 5),
-           "mlups": round(t["mlups"], 1), "kernel": info["kernel"], "vec": info["vec"],
+           "steps": steps, "cells": t["cells"], "ms_per_step": round(ms, 5),
+           "mlups": round(t["mlups"], 1),
+           "diverged_domains": int(bad.sum()), "finite_domains": int((~bad).sum()),
+           "finite_cells": finite_cells, "mlups_finite_domains": round(finite_cells / (ms * 1e3), 1),
+           "bytes_per_step": 2 * Q * 4 * t["cells"],
+           "achieved_gbs": round(2 * Q * 4 * t["cells"] / (ms * 1e-3) / 1e9, 1),
+           "roofline_frac": round(2 * Q * 4 * t["cells"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "kernel": info["kernel"], "vec": info["vec"],
            "blocks_per_cu": info["blocks_per_cu"], "tiles": info["ntiles"], "near_cells": info["nnear"],
            "adjoint": None}
     if adj_steps <= 0:
@@ -217,8 +344,7 @@ def batch_leg(capi, steps: int = 2000, n: int = 512, adj_steps: int = 100):
     J = tape.forward()
     g = tape.backward()
     tm = tape.timing()
-    # tau = 3 U L / Re + 1/2 reaches 0.5024 for small, high-Re cavities, where
-    # BGK is unstable: those diverge in the forward run and are counted here
+    # cavities that diverge in the forward run have a non-finite objective
     stable = np.isfinite(J)
     tape.close()
     b.close()
@@ -244,6 +370,9 @@ def main() -> None:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-aa", action="store_true", help="skip the in-place layout line")
+    ap.add_argument("--no-walled", action="store_true", help="skip the walled (cavity, obstacle channel) leg")
+    ap.add_argument("--walled-modes", default="f32,f16")
+    ap.add_argument("--no-report", action="store_true", help="skip the lattice / fp64 report rows")
     ap.add_argument("--batch-steps", type=int, default=2000, help="timed steps of the 512-cavity batch")
     ap.add_argument("--batch-adjoint-steps", type=int, default=100, help="batched adjoint steps (0: skip)")
     ap.add_argument("--calibrate", action="store_true",
@@ -286,41 +415,45 @@ def main() -> None:
         "data": "synthetic (seeded Taylor-Green vortex + density noise, seed 12345)",
         "config": {"workload": "D3Q19 periodic 256^3 TGV, BGK tau=0.8, shifted f-w stored fp32",
                    "lattice": "D3Q19", "shape": [N, N, N], "tau": TAU, "storage": head_mode,
-                   "steps_per_pass": head["steps_per_pass"], "z_chunk": head["z_chunk"]},
+                   "steps_per_pass": head["steps_per_pass"], "z_chunk": head["z_chunk"],
+                   "mlups_ghost_inclusive": round(head["mlups_ghost_inclusive"], 1)},
+        # achieved = bytes one launch must move (each population read once and
+        # written once: 2*Q*s per cell, also for a launch that advances two
+        # steps) / the event-timed average launch
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "kernel": head["kernel"], "kernel_ms": round(head["kernel_ms"], 5),
                      "steps_per_launch": head["steps_per_pass"],
-                     "alg_bytes_per_launch": head["alg_bytes"],
+                     "bytes_per_launch": head["bytes_per_launch"],
+                     "traffic_over_bytes": round(traffic / head["bytes_per_launch"], 4) if traffic else None,
                      # HBM bytes actually moved per launch over its duration
                      "hbm_gbs_measured": (round(traffic / (head["kernel_ms"] * 1e-3) / 1e9, 1)
                                           if traffic else None),
+                     # 2*Q*s per cell-update over the launch time: a rate of
+                     # step-equivalents, not an HBM rate (a pass moves half of it)
+                     "step_equiv_gbs": round(head["step_equiv_gbs"], 1),
                      "copy_gbs": round(copy["gbs"], 1),
                      "q_plane_copy_gbs": round(qcopy["gbs"], 1),
                      "single_step": {"kernel": s_head["kernel"], "kernel_ms": round(s_head["kernel_ms"], 5),
                                      "achieved": round(s_head["achieved_gbs"], 1),
                                      "frac": round(s_head["achieved_gbs"] / HBM_PEAK_GBS, 4),
                                      "traffic": s_traffic,
-                                     "alg_bytes_per_launch": s_head["alg_bytes"],
+                                     "bytes_per_launch": s_head["bytes_per_launch"],
                                      "frac_of_copy": round(s_head["achieved_gbs"] / copy["gbs"], 4),
                                      "frac_of_q_plane_copy": round(s_head["achieved_gbs"] / qcopy["gbs"], 4),
                                      "mlups": round(s_head["mlups"], 1),
                                      "vec": s_head["vec"], "block": s_head["block"]}},
-        "modes": {m: {"mlups": round(r["mlups"], 1), "kernel": r["kernel"], "kernel_ms": round(r["kernel_ms"], 5),
-                      "achieved_gbs": round(r["achieved_gbs"], 1),
-                      "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                      "bytes_per_cell_step": 2 * Q * BYTES[m],
-                      "single_step": {"mlups": round(single[m]["mlups"], 1),
-                                      "kernel_ms": round(single[m]["kernel_ms"], 5),
-                                      "achieved_gbs": round(single[m]["achieved_gbs"], 1),
-                                      "roofline_frac": round(single[m]["achieved_gbs"] / HBM_PEAK_GBS, 4)}}
-                  for m, r in res.items()},
+        "modes": {m: dict(mode_row(r, m), single_step=mode_row(single[m], m)) for m, r in res.items()},
     }
     if "f16" in res and "f32" in res:
         line["f16_over_f32"] = round(res["f16"]["mlups"] / res["f32"]["mlups"], 3)
         line["f16_over_f32_single_step"] = round(single["f16"]["mlups"] / single["f32"]["mlups"], 3)
+    if not a.no_walled:
+        line["walled"] = walled_leg(capi, min(a.steps, 200), [m for m in a.walled_modes.split(",") if m])
+    if not a.no_report:
+        line["report"] = report_leg(capi, min(a.steps, 100), a.warmup, rho, u)
     if not a.no_aa:
         line["inplace_aa"] = aa_leg(capi, head_mode, min(a.steps, 200), rho, u)
     line["adjoint"] = None if a.no_adjoint else adjoint_leg(capi)

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define LBM_HIP_ABI_VERSION 4
+#define LBM_HIP_ABI_VERSION 5
 
 /* Status codes. */
 #define LBM_OK 0
@@ -260,6 +260,14 @@ typedef struct lbm_batch_info {
     int kernel;        /* LBM_BATCH_ROWS_SEPARATE, LBM_BATCH_ROWS_FUSED or LBM_BATCH_CELLS */
 } lbm_batch_info;
 int lbm_batch_get_info(lbm_batch* b, lbm_batch_info* info);
+/* Divergence flags (extension; the reference has one domain and no such
+ * check): nonfinite (or NULL) receives one int per domain, in creation
+ * order, 1 if a batched step since creation or the last reset read a NaN or
+ * infinite value in one of that domain's fluid cells (the collision's
+ * density and velocity of the incoming values are tested), else 0. Flags are
+ * set inside the step kernels and only ever raised there; reset != 0 clears
+ * them after they are read. Waits for the batch's stream. */
+int lbm_batch_status(lbm_batch* b, int* nonfinite, int reset);
 /* Kernel forms of a batched step. ROWS_*: blocks of threads over (x, y) rows
  * of one z plane, V cells per thread, near-boundary cells in a second launch
  * over their list (SEPARATE) or in the same pass (FUSED). CELLS: 64
@@ -322,14 +330,28 @@ int lbm_get_info(lbm_domain* d, lbm_info* info);
  * divisor of 256; 0 = automatic). Results are the same bits either way. */
 int lbm_set_tuning(lbm_domain* d, int vec, int block_x);
 
-/* Time steps per pass over the field. Where every interior cell is bulk
- * (periodic along all three axes, no boundary cells) and the storage mode is
- * fp32, fp16 or bf16 (16-bit modes: x rows longer than 32 cells), lbm_step
- * can run two steps in one pass: the field is read once and written once for
- * both, with the intermediate state held on chip. Results are the same bits as two single steps. steps_per_pass: 1 or 2
- * (default 2; a domain where the pass does not apply steps singly).
- * z_chunk: planes per block of the pass (0 = automatic). No reference
- * counterpart: a scheduling choice of this implementation. */
+/* Index arithmetic of the cell-tile and in-place kernels: 32 (default) uses
+ * a uniform per-direction base plus a 32-bit byte offset wherever a
+ * direction plane holds less than 4 GiB; 64 forces 64-bit element indices.
+ * Results are the same bits either way (the switch exists to test both
+ * builds). No reference counterpart. */
+int lbm_set_index_width(lbm_domain* d, int bits);
+
+/* Time steps per pass over the field. With planes under 4 GiB, lbm_step
+ * runs two steps in one pass: the field is read once and written once for
+ * both, with the intermediate state held on chip. This applies to bulk-only
+ * domains (periodic along all three axes) in fp32, fp16 or bf16 storage
+ * (16-bit modes: x rows longer than 32 cells), and to fp32 domains with
+ * NoSlip, MovingWall, Inflow, Outflow and Pressure boundaries and solid
+ * cells, where the first step after initialisation or lbm_set_pdf stays a
+ * single step. A domain with a FreeSlip condition, a Brinkman field, the
+ * in-place layout or a vector width set by lbm_set_tuning steps singly.
+ * Results on fluid cells are the same bits as single steps. steps_per_pass:
+ * 0 (default, automatic): two steps per pass where that measured faster,
+ * which leaves D3Q27 domains with boundary cells stepping singly; 2: two
+ * steps per pass wherever it applies; 1: single steps. z_chunk: planes per
+ * block of the pass (0 = automatic). No reference counterpart: a scheduling
+ * choice of this implementation. */
 int lbm_set_time_blocking(lbm_domain* d, int steps_per_pass, int z_chunk);
 
 #ifdef __cplusplus

@@ -86,6 +86,7 @@ SIGNATURES = [
     ("lbm_batch_time_steps", c_int, [c_void_p, c_uint64, POINTER(c_double)]),
     ("lbm_batch_get_info", c_int, [c_void_p, c_void_p]),
     ("lbm_batch_set_config", c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    ("lbm_batch_status", c_int, [c_void_p, POINTER(c_int), c_int]),
     ("lbm_batch_adj_create", c_int, [c_void_p, c_uint64, c_uint64, POINTER(c_void_p)]),
     ("lbm_batch_adj_destroy", c_int, [c_void_p]),
     ("lbm_batch_adj_forward", c_int, [c_void_p, POINTER(c_double)]),
@@ -94,6 +95,7 @@ SIGNATURES = [
     ("lbm_get_info", c_int, [c_void_p, POINTER(Info)]),
     ("lbm_set_tuning", c_int, [c_void_p, c_int, c_int]),
     ("lbm_set_time_blocking", c_int, [c_void_p, c_int, c_int]),
+    ("lbm_set_index_width", c_int, [c_void_p, c_int]),
 ]
 
 _lib = None
@@ -147,6 +149,15 @@ def _dptr(a: np.ndarray):
     return a.ctypes.data_as(POINTER(c_double))
 
 
+def _f64(a, size: int, what: str) -> np.ndarray:
+    """Contiguous flat fp64 copy of a, checked to hold exactly `size` values:
+    the C ABI reads that many from the pointer."""
+    v = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if v.size != size:
+        raise ValueError(f"{what}: expected {size} values, got {v.size}")
+    return v
+
+
 class Domain:
     """One single-device domain (include/lbm_hip.h: lbm_domain)."""
 
@@ -197,16 +208,16 @@ class Domain:
 
     def set_solid_mask(self, fluid_mask: np.ndarray) -> None:
         m = np.ascontiguousarray(fluid_mask, dtype=np.uint8).reshape(-1)
-        assert m.size == self.xl * self.yl * self.zl
+        if m.size != self.xl * self.yl * self.zl:
+            raise ValueError(f"fluid mask: expected {self.xl * self.yl * self.zl} values, got {m.size}")
         _check(load().lbm_set_solid_mask(self._h, m.ctypes.data_as(POINTER(c_uint8))))
 
     def set_brinkman(self, alpha, g=(0.0, 0.0, 0.0)) -> None:
-        gv = np.ascontiguousarray(g, dtype=np.float64)
+        gv = _f64(g, 3, "body acceleration")
         if alpha is None:
             _check(load().lbm_set_brinkman(self._h, None, _dptr(gv)))
             return
-        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1)
-        assert a.size == self.xl * self.yl * self.zl
+        a = _f64(alpha, self.xl * self.yl * self.zl, "porosity")
         _check(load().lbm_set_brinkman(self._h, _dptr(a), _dptr(gv)))
 
     def finalize(self) -> None:
@@ -236,15 +247,13 @@ class Domain:
         return out
 
     def set_pdf(self, f: np.ndarray) -> None:
-        a = np.ascontiguousarray(f, dtype=np.float64)
-        assert a.size == np.prod(self.ghost_shape) * self.q
+        a = _f64(f, int(np.prod(self.ghost_shape)) * self.q, "pdf field")
         _check(load().lbm_set_pdf(self._h, _dptr(a)))
 
     def init_equilibrium(self, rho: np.ndarray, u: np.ndarray) -> None:
-        r = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
-        v = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
         n = self.xl * self.yl * self.zl
-        assert r.size == n and v.size == 3 * n
+        r = _f64(rho, n, "density")
+        v = _f64(u, 3 * n, "velocity")
         _check(load().lbm_init_equilibrium(self._h, _dptr(r), _dptr(v)))
 
     def moments(self):
@@ -267,9 +276,16 @@ class Domain:
     def set_tuning(self, vec: int = 0, block_x: int = 0) -> None:
         _check(load().lbm_set_tuning(self._h, vec, block_x))
 
-    def set_time_blocking(self, steps_per_pass: int = 2, z_chunk: int = 0) -> None:
-        """Two steps per pass for bulk-only (fully periodic) domains in
-        fp32/fp16/bf16 storage (default), or 1 for single steps."""
+    def set_index_width(self, bits: int = 32) -> None:
+        """32: 32-bit byte offsets where a plane is under 4 GiB (default);
+        64: 64-bit element indices in the cell-tile and in-place kernels."""
+        _check(load().lbm_set_index_width(self._h, bits))
+
+    def set_time_blocking(self, steps_per_pass: int = 0, z_chunk: int = 0) -> None:
+        """0 (default): two steps per pass where that measured faster
+        (bulk-only domains in fp32/fp16/bf16; fp32 D3Q15/D3Q19 domains with
+        boundary cells); 2: two steps per pass wherever it applies (also fp32
+        D3Q27 domains with boundary cells); 1: single steps."""
         _check(load().lbm_set_time_blocking(self._h, steps_per_pass, z_chunk))
 
 
@@ -294,8 +310,7 @@ class Tape:
         d = AdjDesc()
         d.objective, d.plane_x, d.nsteps, d.checkpoint_every = objective, plane_x, nsteps, checkpoint_every
         if target_u is not None:
-            self._target = np.ascontiguousarray(target_u, dtype=np.float64).reshape(-1)
-            assert self._target.size == dom.yl * dom.zl * 3
+            self._target = _f64(target_u, dom.yl * dom.zl * 3, "target velocity plane")
             d.target_u = _dptr(self._target)
         h = c_void_p()
         _check(load().lbm_adj_create(dom._h, byref(d), byref(h)))
@@ -316,8 +331,7 @@ class Tape:
     def _alpha(self, alpha):
         if alpha is None:
             return None, None
-        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1)
-        assert a.size == self.dom.xl * self.dom.yl * self.dom.zl
+        a = _f64(alpha, self.dom.xl * self.dom.yl * self.dom.zl, "porosity")
         return a, _dptr(a)
 
     def forward(self, alpha=None) -> float:
@@ -335,7 +349,7 @@ class Tape:
 
     def jvp(self, alpha, dalpha):
         a, p = self._alpha(alpha)
-        da = np.ascontiguousarray(dalpha, dtype=np.float64).reshape(-1)
+        da = _f64(dalpha, self.dom.xl * self.dom.yl * self.dom.zl, "porosity direction")
         j, dj = c_double(), c_double()
         _check(load().lbm_jvp(self._h, p, _dptr(da), byref(j), byref(dj)))
         return j.value, dj.value
@@ -396,6 +410,14 @@ class Batch:
     def set_config(self, kernel: int = 2, vec: int = 1, bx_policy: int = 0, blocks_per_cu: int = 8) -> None:
         """kernel: BATCH_ROWS_SEPARATE, BATCH_ROWS_FUSED or BATCH_CELLS."""
         _check(load().lbm_batch_set_config(self._h, kernel, vec, bx_policy, blocks_per_cu))
+
+    def status(self, reset: bool = False) -> np.ndarray:
+        """Per-domain divergence flags (bool array in domain order): True if a
+        batched step read a NaN or infinite value in that domain's fluid
+        cells since creation or the last reset."""
+        out = np.zeros(len(self.domains), dtype=np.int32)
+        _check(load().lbm_batch_status(self._h, out.ctypes.data_as(POINTER(c_int)), 1 if reset else 0))
+        return out != 0
 
     def info(self) -> dict:
         i = BatchInfo()

@@ -19,6 +19,7 @@ struct BatchView {
     FieldLayout L;
     void* buf[2];
     const uint8_t* kind;
+    const uint32_t* lmask;
     const Cond* conds;
     double tau, omega;
     int bx_log2;  // row tiles: threads along x per tile = 1 << bx_log2 (of 256)
@@ -58,6 +59,9 @@ struct BatchArgs {
     int materialised;  // the step pulls stored boundary values
     int freeslip;      // some domain has a FreeSlip condition (cell tiles built with its rule)
     int idx32;         // every domain's plane is under 4 GiB (cell tiles use 32-bit byte offsets)
+    // one flag per domain, set to 1 by a step in which a fluid cell of the
+    // domain read a NaN or infinite value (never cleared by the kernels)
+    int* status;
 };
 
 struct BatchOps {

@@ -24,6 +24,7 @@ struct BatchAdjView {
     int64_t slot_elems;   // Q * plane
     const void* gfinal;   // the forward run's final field (the domain's buffer)
     const uint8_t* kind;
+    const uint32_t* lmask;
     const Cond* conds;
     double tau, omega;
     double* gw;           // 3 doubles per element: dJ/du_wall accumulated per cell

@@ -123,7 +123,7 @@ int make_plan(lbm_domain* d) {
     p.rxl = 1.0 / static_cast<double>(d->L.xl);
     p.freeslip = false;
     for (const HostCond& c : d->conds) p.freeslip |= c.kind == LBM_BC_FREESLIP;
-    p.idx32 = d->L.plane * static_cast<int64_t>(d->esize) < (int64_t{1} << 32);
+    p.idx32 = !d->force64 && d->L.plane * static_cast<int64_t>(d->esize) < (int64_t{1} << 32);
     p.cells = d->tune_vec == 0 && (d->nb > 0 || d->compact) &&
               p.nblocks_cells <= 0x7fffffffLL && (plane_cells + 63) / 64 <= 0x7fffffffLL;
     if (!p.cells && d->compact && V > 1)
@@ -134,16 +134,30 @@ int make_plan(lbm_domain* d) {
     p.tb_nty = (d->L.yl + TB_TY - 1) / TB_TY;
     const int64_t tiles = static_cast<int64_t>(p.tb_ntx) * p.tb_nty;
     int ncu = 0;
-    if (o->step2 && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d->desc.device) != hipSuccess)
+    if ((o->step2 || o->step2w) &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d->desc.device) != hipSuccess)
         ncu = 0;
+    // Which pass would run: k_bulk2 where every interior cell is bulk, k_wall2
+    // where there are boundary cells (not FreeSlip, whose rule reads other
+    // cells than the one it serves; no Brinkman field).
+    const bool walled = d->nb > 0 || p.flags;
+    const bool pass_ok = d->tune_pass != 1 && !d->aa &&
+                         d->L.plane * static_cast<int64_t>(d->esize) < (int64_t{1} << 32) &&
+                         (d->esize >= 4 || d->L.xl > 32);
     // Default z chunks: a block marches through its zc planes plus one more
     // at each end, so the pass takes about rounds x (zc + 2) plane times,
     // with rounds = ceil(blocks / resident blocks). The chunk count that
     // minimises it is taken, among chunks of at most 32 planes (longer ones
     // measured slower on planes of many tiles).
     int zc = d->tune_zc;
-    if (!zc && o->step2) {
-        const int64_t resident = std::max<int64_t>(1, static_cast<int64_t>(ncu) * o->step2_blocks_per_cu(p.brink));
+    const bool use_w = walled && o->step2w != nullptr;
+    // rule families the walled pass is built with: NoSlip (+ MovingWall) or all
+    p.wrules = RULE_MOVINGWALL;
+    for (const HostCond& c : d->conds)
+        if (c.kind != LBM_BC_NOSLIP && c.kind != LBM_BC_MOVINGWALL) p.wrules = RULES_ALL;
+    if (!zc && (use_w ? o->step2w != nullptr : o->step2 != nullptr)) {
+        const int per_cu = use_w ? o->step2w_blocks_per_cu(p.wrules) : o->step2_blocks_per_cu(p.brink);
+        const int64_t resident = std::max<int64_t>(1, static_cast<int64_t>(ncu) * per_cu);
         const int64_t zl = d->L.zl;
         int64_t best = -1, best_cost = 0;
         for (int64_t c = (zl + 31) / 32; c <= zl; ++c) {
@@ -166,10 +180,13 @@ int make_plan(lbm_domain* d) {
     // is bound by instruction issue: on rows of 32 cells or fewer most lanes
     // of its 64-cell tile rows idle, and single steps (cell tiles over packed
     // rows) measured faster there.
-    p.pairs = d->tune_pass == 2 && o->step2 != nullptr && !d->aa && d->nb == 0 && !p.flags && p.has_bulk &&
-              p.tb_nblocks <= 0x7fffffffLL &&
-              d->L.plane * static_cast<int64_t>(d->esize) < (int64_t{1} << 32) &&
-              (d->esize >= 4 || d->L.xl > 32);
+    p.pairs = pass_ok && o->step2 != nullptr && !walled && p.has_bulk && p.tb_nblocks <= 0x7fffffffLL;
+    // Automatic choice (tune_pass 0): the walled pass on D3Q15 and D3Q19. On
+    // D3Q27 its step 1 holds 27 values per patch cell across the plane and
+    // measured slower than single steps (Q27 channel around a sphere, fp32).
+    p.wpairs = pass_ok && o->step2w != nullptr && walled && d->tune_vec == 0 && !p.freeslip && !p.brink &&
+               d->n_fluid > 0 && (d->tune_pass == 2 || o->q <= 19) &&
+               p.tb_nblocks <= 0x7fffffffLL;
     return LBM_OK;
 }
 
@@ -177,6 +194,7 @@ void free_device(lbm_domain* d) {
     (void)hipFree(d->buf[0]);
     (void)hipFree(d->buf[1]);
     (void)hipFree(d->d_kind);
+    (void)hipFree(d->d_lmask);
     (void)hipFree(d->d_blist);
     (void)hipFree(d->d_conds);
     (void)hipFree(d->d_alpha);
@@ -184,6 +202,7 @@ void free_device(lbm_domain* d) {
     (void)hipFree(d->d_out4);
     d->buf[0] = d->buf[1] = nullptr;
     d->d_kind = nullptr;
+    d->d_lmask = nullptr;
     d->d_blist = nullptr;
     d->d_conds = nullptr;
     d->d_alpha = nullptr;
@@ -451,6 +470,10 @@ int lbm_finalize(lbm_domain* d) {
         c.kind = uniq[j].kind;
         for (int k = 0; k < 3; ++k) c.vel[k] = uniq[j].vel[k];
         c.rho = uniq[j].rho;
+        for (int q = 0; q < Q; ++q) {
+            c.cu64[q] = lat.c[q][0] * c.vel[0] + lat.c[q][1] * c.vel[1] + lat.c[q][2] * c.vel[2];
+            c.cu32[q] = static_cast<float>(c.cu64[q]);
+        }
         if (c.kind == LBM_BC_INFLOW) {
             host_feq(lat, c.rho, c.vel, c.feq);
             for (int q = 0; q < Q; ++q) {
@@ -461,8 +484,10 @@ int lbm_finalize(lbm_domain* d) {
         }
     }
 
-    // Cell kinds in the padded device layout, and the NEAR list.
+    // Cell kinds in the padded device layout, the NEAR list, and the link
+    // masks of NEAR cells (bit q: the source x - c_q is not fluid).
     std::vector<uint8_t> kind(static_cast<size_t>(L.plane), KIND_INERT);
+    std::vector<uint32_t> lmask(static_cast<size_t>(L.plane), 0u);
     std::vector<CellCoord> blist;
     uint64_t nfluid = 0, nbulk = 0, nonbulk_interior = 0;
     for (int z = 0; z < d->nz; ++z)
@@ -478,15 +503,16 @@ int lbm_finalize(lbm_domain* d) {
                 }
                 if (!in) continue;  // unassigned ghost: INERT
                 ++nfluid;
-                bool near = false;
-                for (int q = 0; q < Q && !near; ++q) {
+                uint32_t m = 0;
+                for (int q = 0; q < Q; ++q) {
                     const int64_t sx = d->wrapc(x - lat.c[q][0], 0);
                     const int64_t sy = d->wrapc(y - lat.c[q][1], 1);
                     const int64_t sz = d->wrapc(z - lat.c[q][2], 2);
-                    near = d->handler[d->hidx(sx, sy, sz)] != 0;
+                    if (d->handler[d->hidx(sx, sy, sz)] != 0) m |= 1u << q;
                 }
-                if (near) {
+                if (m) {
                     kind[e] = KIND_NEAR;
+                    lmask[e] = m;
                     blist.push_back(CellCoord{x, y, z, 0});
                     ++nonbulk_interior;
                 } else {
@@ -496,6 +522,12 @@ int lbm_finalize(lbm_domain* d) {
             }
 
     HIPCHK(hipMemcpyAsync(d->d_kind, kind.data(), kind.size(), hipMemcpyHostToDevice, d->stream));
+    if (!d->d_lmask) {
+        HIPCHK(hipMalloc(&d->d_lmask, lmask.size() * sizeof(uint32_t)));
+        d->device_bytes += lmask.size() * sizeof(uint32_t);
+    }
+    HIPCHK(hipMemcpyAsync(d->d_lmask, lmask.data(), lmask.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          d->stream));
     d->nb = static_cast<int64_t>(blist.size());
     if (d->nb) {
         HIPCHK(hipMalloc(&d->d_blist, blist.size() * sizeof(CellCoord)));
@@ -559,6 +591,23 @@ static int run_steps(lbm_domain* d, uint64_t nsteps, EventLog* log) {
         return LBM_OK;
     }
     uint64_t t = 0;
+    if (d->plan.wpairs) {
+        // the first step after set-up pulls stored boundary values: single
+        if (d->materialised && t < nsteps) {
+            const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur ^ 1], 1);
+            hipError_t e = d->ops->step(a, d->plan, d->stream, log);
+            if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e));
+            d->cur ^= 1;
+            d->materialised = false;
+            ++t;
+        }
+        for (; t + 2 <= nsteps; t += 2) {
+            const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur ^ 1], 0);
+            hipError_t e = d->ops->step2w(a, d->plan, d->stream, log);
+            if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("two-step launch: ") + hipGetErrorString(e));
+            d->cur ^= 1;
+        }
+    }
     if (d->plan.pairs)
         for (; t + 2 <= nsteps; t += 2) {
             // (every source is an interior cell, so the flag has no effect here;
@@ -579,9 +628,27 @@ static int run_steps(lbm_domain* d, uint64_t nsteps, EventLog* log) {
     return LBM_OK;
 }
 
+}  // extern "C"
+
+namespace lbm_amd {
+
+// Boundary cells of the current field <- the values the reference's boundary
+// pass leaves there (include/domain.hpp:157-165): the rule value of every
+// link that points into a fluid cell, from that cell's stored values. The
+// steps compute those values where they are pulled and do not store them;
+// readers of whole cells (lbm_get_pdf, lbm_moments) see the reference's
+// field. In the in-place layout the boundary cells already hold them.
+int materialise_boundaries(lbm_domain* d) {
+    if (d->finalized && !d->materialised && !d->aa) {
+        const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur], 0);
+        HIPCHK(d->ops->materialise(a, d->nx, d->ny, d->nz, d->stream));
+    }
+    return LBM_OK;
+}
+
 // In-place layout: bring the field to the natural representation (the
 // readers' and writers' layout) if the last step left it swapped.
-static int aa_settle(lbm_domain* d) {
+int aa_settle(lbm_domain* d) {
     if (!d->aa || d->aa_phase == 0) return LBM_OK;
     const StepArgs a = d->args(d->buf[0], d->buf[0], 0);
     const hipError_t e = d->ops->aa_swap(a, d->plan, d->stream);
@@ -589,6 +656,10 @@ static int aa_settle(lbm_domain* d) {
     d->aa_phase = 0;
     return LBM_OK;
 }
+
+}  // namespace lbm_amd
+
+extern "C" {
 
 int lbm_step(lbm_domain* d, uint64_t nsteps) {
     if (int rc = check_handle(d)) return rc;
@@ -634,11 +705,7 @@ int lbm_get_pdf(lbm_domain* d, double* out) {
     if (!out) return fail(LBM_ERR_ARG, "null output");
     if (!d->finalized && !d->materialised) return fail(LBM_ERR_STATE, "inconsistent state");
     if (int rc = aa_settle(d)) return rc;
-    // (in the in-place layout the boundary cells already hold the rule values)
-    if (d->finalized && !d->materialised && !d->aa) {
-        const StepArgs a = d->args(d->buf[d->cur], d->buf[d->cur], 0);
-        HIPCHK(d->ops->materialise(a, d->nx, d->ny, d->nz, d->stream));
-    }
+    if (int rc = materialise_boundaries(d)) return rc;
     const size_t bytes = static_cast<size_t>(d->ncell) * d->Q * sizeof(double);
     double* dev = nullptr;
     HIPCHK(hipMalloc(&dev, bytes));
@@ -691,6 +758,7 @@ int lbm_init_equilibrium(lbm_domain* d, const double* rho, const double* u) {
 int lbm_moments(lbm_domain* d, double* rho, double* u) {
     if (int rc = check_handle(d)) return rc;
     if (int rc = aa_settle(d)) return rc;
+    if (int rc = materialise_boundaries(d)) return rc;
     const size_t n = static_cast<size_t>(d->L.xl) * d->L.yl * d->L.zl;
     double* dev = nullptr;
     HIPCHK(hipMalloc(&dev, 4 * n * sizeof(double)));
@@ -818,14 +886,23 @@ int lbm_get_info(lbm_domain* d, lbm_info* info) {
     info->nconds = d->nconds_dev;
     info->cell_tiles = (d->plan.cells || d->aa) ? 1 : 0;
     info->layout = d->aa ? LBM_LAYOUT_AA : LBM_LAYOUT_AB;
-    info->steps_per_pass = d->plan.pairs ? 2 : 1;
+    info->steps_per_pass = (d->plan.pairs || d->plan.wpairs) ? 2 : 1;
     info->z_chunk = d->plan.tb_zc;
+    return LBM_OK;
+}
+
+int lbm_set_index_width(lbm_domain* d, int bits) {
+    if (int rc = check_handle(d)) return rc;
+    if (bits != 32 && bits != 64) return fail(LBM_ERR_ARG, "index width must be 32 or 64");
+    d->force64 = bits == 64;
+    if (d->finalized) return make_plan(d);
     return LBM_OK;
 }
 
 int lbm_set_time_blocking(lbm_domain* d, int steps_per_pass, int z_chunk) {
     if (int rc = check_handle(d)) return rc;  // (planning queries the domain's device)
-    if (steps_per_pass != 1 && steps_per_pass != 2) return fail(LBM_ERR_ARG, "steps_per_pass must be 1 or 2");
+    if (steps_per_pass < 0 || steps_per_pass > 2)
+        return fail(LBM_ERR_ARG, "steps_per_pass must be 0 (automatic), 1 or 2");
     if (z_chunk < 0) return fail(LBM_ERR_ARG, "z_chunk must be >= 0");
     const int op = d->tune_pass, oz = d->tune_zc;
     d->tune_pass = steps_per_pass;

@@ -93,10 +93,11 @@ int lbm_adj_create(lbm_domain* d, const lbm_adj_desc* desc, lbm_tape** out) {
     if (!d->finalized) return fail(LBM_ERR_STATE, "finalize the domain before creating a tape");
     if (d->aa) return fail(LBM_ERR_UNSUPPORTED, "the adjoint needs the two-buffer layout");
     const AdjOps* ops = adj_ops(d->Q, d->mode);
-    if (!ops) return fail(LBM_ERR_UNSUPPORTED, "adjoint needs D3Q19 with F32 or F64S storage");
+    if (!ops) return fail(LBM_ERR_UNSUPPORTED, "adjoint needs F32 or F64S storage");
     for (const HostCond& c : d->conds)
-        if (c.kind != LBM_BC_NOSLIP && c.kind != LBM_BC_MOVINGWALL && c.kind != LBM_BC_INFLOW)
-            return fail(LBM_ERR_UNSUPPORTED, "adjoint supports NoSlip, MovingWall and Inflow boundaries");
+        if (c.kind == LBM_BC_FREESLIP)
+            return fail(LBM_ERR_UNSUPPORTED,
+                        "adjoint supports NoSlip, MovingWall, Inflow, Outflow and Pressure boundaries");
     if (desc->objective != LBM_OBJ_PLANE_VELOCITY_MISFIT && desc->objective != LBM_OBJ_KINETIC_ENERGY)
         return fail(LBM_ERR_ARG, "unknown objective");
     if (desc->objective == LBM_OBJ_PLANE_VELOCITY_MISFIT &&

@@ -37,6 +37,7 @@ struct lbm_batch {
     BatchView* d_views = nullptr;
     TileRef* d_tiles = nullptr;
     NearRef* d_near = nullptr;
+    int* d_status = nullptr;    // per-domain non-finite flags (BatchArgs::status)
     int64_t ntiles = 0, nnear = 0;
     uint64_t ncells = 0;
     hipGraphExec_t exec = nullptr;
@@ -84,6 +85,7 @@ int configure(lbm_batch* b, int V, int bx_policy, int bpc, int kernel) {
         v.buf[0] = d->buf[0];
         v.buf[1] = d->buf[1];
         v.kind = d->d_kind;
+        v.lmask = d->d_lmask;
         v.conds = d->d_conds;
         v.tau = d->desc.tau;
         v.omega = 1.0 / d->desc.tau;
@@ -161,6 +163,7 @@ hipError_t enqueue(lbm_batch* b, uint64_t nsteps, int parity, int mat) {
     a.nnear = b->nnear;
     a.freeslip = b->any_freeslip ? 1 : 0;
     a.idx32 = b->all_idx32 ? 1 : 0;
+    a.status = b->d_status;
     // blocks that give every tile its own block (cell tiles: 4 wavefronts a block)
     const int64_t full = std::max<int64_t>(1, b->kernel == BK_CELLS ? (b->ntiles + 3) / 4 : b->ntiles);
     const int grid_bulk = static_cast<int>(b->bpc ? std::min<int64_t>(full, static_cast<int64_t>(b->ncu) * b->bpc)
@@ -225,6 +228,13 @@ int lbm_batch_create(lbm_domain* const* domains, int n, lbm_batch** out) {
         delete b;
         return fail(LBM_ERR_HIP, "hipStreamCreate failed");
     }
+    if (hipMalloc(&b->d_status, sizeof(int) * static_cast<size_t>(n)) != hipSuccess ||
+        hipMemset(b->d_status, 0, sizeof(int) * static_cast<size_t>(n)) != hipSuccess) {
+        (void)hipFree(b->d_status);
+        (void)hipStreamDestroy(b->stream);
+        delete b;
+        return fail(LBM_ERR_ALLOC, "batch status flags");
+    }
     // start from the domains' own cells-per-thread setting
     int V0 = ops->vecs[0];
     for (int i = 0; i < 4 && ops->vecs[i]; ++i)
@@ -232,6 +242,7 @@ int lbm_batch_create(lbm_domain* const* domains, int n, lbm_batch** out) {
     int rc = configure(b, V0, 0, 8, BK_CELLS);
     if (rc) {
         free_lists(b);
+        (void)hipFree(b->d_status);
         (void)hipStreamDestroy(b->stream);
         delete b;
         return rc;
@@ -246,6 +257,7 @@ int lbm_batch_destroy(lbm_batch* b) {
     (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
     free_lists(b);
+    (void)hipFree(b->d_status);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return LBM_OK;
@@ -343,6 +355,9 @@ int lbm_batch_tune(lbm_batch* b, int trial_steps, double* best_ms) {
         saved_cur[i] = b->doms[i]->cur;
         saved_mat[i] = b->doms[i]->materialised;
     }
+    // the trials may also raise non-finite flags: keep the ones the batch had
+    std::vector<int> saved_status(nd);
+    HIPCHK(hipMemcpy(saved_status.data(), b->d_status, sizeof(int) * nd, hipMemcpyDeviceToHost));
     Events ev(2);
     // Kernels are compiled without FMA contraction, so a cell's arithmetic is
     // the same for every V, tile shape and grid size: the search changes
@@ -409,6 +424,8 @@ int lbm_batch_tune(lbm_batch* b, int trial_steps, double* best_ms) {
     {
         const hipError_t e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess && !rc) rc = fail(LBM_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+        const hipError_t e2 = hipMemcpy(b->d_status, saved_status.data(), sizeof(int) * nd, hipMemcpyHostToDevice);
+        if (e2 != hipSuccess && !rc) rc = fail(LBM_ERR_HIP, std::string("restoring status flags: ") + hipGetErrorString(e2));
     }
     for (void* p : saved)
         if (p) (void)hipFree(p);
@@ -434,6 +451,20 @@ int lbm_batch_set_config(lbm_batch* b, int kernel, int vec, int bx_policy, int b
     HIPCHK(hipStreamSynchronize(b->stream));
     drop_graph(b);
     return configure(b, vec, bx_policy, blocks_per_cu, kernel);
+}
+
+int lbm_batch_status(lbm_batch* b, int* nonfinite, int reset) {
+    if (!b) return fail(LBM_ERR_ARG, "null batch");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const size_t n = b->doms.size();
+    if (nonfinite) HIPCHK(hipMemcpy(nonfinite, b->d_status, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (reset) {
+        // outside any captured graph: the replayed steps only ever set flags
+        HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(int) * n, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    return LBM_OK;
 }
 
 int lbm_batch_get_info(lbm_batch* b, lbm_batch_info* info) {
@@ -554,6 +585,7 @@ int lbm_batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_ever
         v.slot_elems = static_cast<int64_t>(d->field_bytes / d->esize);
         v.gfinal = d->buf[d->cur];
         v.kind = d->d_kind;
+        v.lmask = d->d_lmask;
         v.conds = d->d_conds;
         v.tau = d->desc.tau;
         v.omega = 1.0 / d->desc.tau;

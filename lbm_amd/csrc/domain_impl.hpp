@@ -135,6 +135,7 @@ struct lbm_domain {
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;
     uint8_t* d_kind = nullptr;
+    uint32_t* d_lmask = nullptr;  // per element: non-fluid sources of a NEAR cell (bit q)
     CellCoord* d_blist = nullptr;
     int64_t nb = 0;
     Cond* d_conds = nullptr;
@@ -150,7 +151,8 @@ struct lbm_domain {
     hipStream_t stream = nullptr;
     StepPlan plan{};
     int tune_vec = 0, tune_bx = 0;
-    int tune_pass = 2, tune_zc = 0;  // steps per pass, planes per block of the pass
+    int tune_pass = 0, tune_zc = 0;  // steps per pass (0: automatic), planes per block of the pass
+    bool force64 = false;            // 64-bit element indices even where a plane is under 4 GiB
     double* d_partial = nullptr;
     double* d_out4 = nullptr;
     int nred = 0;
@@ -174,6 +176,7 @@ struct lbm_domain {
         a.src = src;
         a.dst = dst;
         a.kind = d_kind;
+        a.lmask = d_lmask;
         a.blist = d_blist;
         a.nb = nb;
         a.conds = d_conds;
@@ -208,6 +211,13 @@ inline std::vector<char> alpha_in_layout(const lbm_domain* d, const double* alph
             }
     return a;
 }
+
+// In-place layout: brings the field back to the natural representation if
+// the last step left it swapped (defined in capi.hip).
+int aa_settle(lbm_domain* d);
+// Writes the boundary links' rule values into the current field, as the
+// reference's boundary pass leaves them (defined in capi.hip).
+int materialise_boundaries(lbm_domain* d);
 
 inline int check_handle(lbm_domain* d) {
     if (!d) return fail(LBM_ERR_ARG, "null domain handle");

@@ -37,6 +37,11 @@ enum : int {
     BC_PRESSURE = 6,
 };
 
+// Rule families a kernel is built with (RULES): NoSlip is always there; a
+// condition of a family left out would be evaluated as NoSlip, so the host
+// leaves a family out only when the domain has no condition of it.
+enum : int { RULE_MOVINGWALL = 1, RULE_INFLOW = 2, RULE_OUTFLOW = 4, RULES_ALL = 7 };
+
 struct Cond {
     int kind;
     int pad;
@@ -46,6 +51,10 @@ struct Cond {
     // and the same shifted by the weights (feq_q - w_q) for the shifted modes.
     double feq[27];
     double geq[27];
+    // MovingWall: c_q . u_w evaluated in fp64 as c_qx u_x + c_qy u_y + c_qz u_z
+    // (left to right), and that value rounded to fp32, for the shifted modes.
+    double cu64[27];
+    float cu32[27];
 };
 
 struct CellCoord {
@@ -66,6 +75,9 @@ struct StepArgs {
     const void* src;
     void* dst;
     const uint8_t* kind;
+    // per element: bit q set when the cell is fluid and its source x - c_q is
+    // not (the links a near-boundary cell takes from boundary rules)
+    const uint32_t* lmask;
     const CellCoord* blist;  // cells of kind NEAR
     int64_t nb;
     const Cond* conds;

@@ -84,6 +84,18 @@ __device__ __forceinline__ void store_stream1(T* p, T v) {
 #endif
 }
 
+template <class T>
+__device__ __forceinline__ void store_stream1(__attribute__((address_space(1))) T* p, T v) {
+#if LBM_NT_STORE
+    using R = typename RawOf<sizeof(T)>::type;
+    R r;
+    __builtin_memcpy(&r, &v, sizeof(R));
+    __builtin_nontemporal_store(r, reinterpret_cast<__attribute__((address_space(1))) R*>(p));
+#else
+    *p = v;
+#endif
+}
+
 // Element indices within a plane: 32-bit where a plane holds less than
 // 4 GiB (B32), so that an access is a uniform direction-plane base plus one
 // 32-bit byte offset per lane; 64-bit otherwise.
@@ -96,16 +108,51 @@ struct ElemIdx<true> {
     using type = uint32_t;
 };
 
-// Element e of direction plane q of field m.
+// Objects in the global address space. Every field, kind array and
+// condition table lives in device (global) memory, but where a kernel reads
+// their pointers from a table in memory (the batched kernels) the compiler
+// cannot prove it and addresses them with flat instructions, which take no
+// scalar base and wait on the LDS counter too. Accesses through Glob<T>
+// references are global loads and stores in every kernel.
+template <class T>
+using Glob = __attribute__((address_space(1))) T;
+
+template <class T>
+__device__ __forceinline__ Glob<T>* glob(T* p) {
+    return (Glob<T>*)p;
+}
+
+// Dual-number fields (the fp64 tangent run) keep generic references: a
+// class with user-declared constructors is not copied out of an
+// address-space-qualified object.
+template <class T>
+struct IsDual : std::false_type {};
+template <class U>
+struct IsDual<Dual<U>> : std::true_type {};
+template <class T>
+using PlaneElem = std::conditional_t<IsDual<std::remove_const_t<T>>::value, T, Glob<T>>;
+
+// Element e of direction plane q of field m (global memory).
 template <bool B32, class T, class I>
-__device__ __forceinline__ T& plane_ref(T* m, int q, int64_t plane, I e) {
+__device__ __forceinline__ PlaneElem<T>& plane_ref(T* m, int q, int64_t plane, I e) {
     using Byte = std::conditional_t<std::is_const_v<T>, const char, char>;
+    using E = PlaneElem<T>;
+    using EB = std::conditional_t<IsDual<std::remove_const_t<T>>::value, Byte, Glob<Byte>>;
+    E* base = (E*)(m + q * plane);
     if constexpr (B32)
-        return *reinterpret_cast<T*>(reinterpret_cast<Byte*>(m + q * plane) +
+        return *reinterpret_cast<E*>(reinterpret_cast<EB*>(base) +
                                      static_cast<uint32_t>(e) * static_cast<uint32_t>(sizeof(T)));
     else
-        return m[q * plane + e];
+        return base[e];
 }
+
+// Cell kind at element e, and condition i of a domain's table.
+template <class I>
+__device__ __forceinline__ uint8_t kind_at(const uint8_t* kind, I e) {
+    return glob(kind)[e];
+}
+using GCond = Glob<const Cond>;
+__device__ __forceinline__ const GCond& cond_at(const Cond* conds, int i) { return glob(conds)[i]; }
 
 // cell_elem in the index type of plane_ref. With B32 every element index of
 // the field is below 2^32 and all terms are non-negative, so 32-bit
@@ -133,16 +180,17 @@ __device__ __forceinline__ int wrap_if(int v, int l, int per) {
 // Source rows are wrapped unconditionally: along a periodic axis that is the
 // periodic image, and along a walled axis any cell next to the wall pulls
 // from a ghost cell, is NEAR and is masked out before its values are used.
-// V cells (x0 .. x0+V-1, y, z) of one thread.
+// V cells (x0 .. x0+V-1, y, z) of one thread. Returns false when a cell it
+// updated read a NaN or infinite value (finite_calc of the collision's base).
 template <class L, Mode M, int V, bool FLAGS, bool BRINK>
-__device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int z,
+__device__ __forceinline__ bool bulk_cells(const StepArgs& a, int x0, int y, int z,
                                            bool* near_lanes = nullptr) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
     constexpr int Q = L::Q;
     const FieldLayout& Ly = a.L;
-    if (x0 > Ly.xl || y > Ly.yl) return;
+    if (x0 > Ly.xl || y > Ly.yl) return true;
 
     const int64_t e0 = cell_elem(Ly, x0, y, z);
     bool act[V];
@@ -204,12 +252,12 @@ __device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int
             }
         }
 #pragma unroll
-        for (int i = 0; i < V; ++i) f[i][q] = IO::template to_calc<q>(v[i]);
+        for (int i = 0; i < V; ++i) f[i][q] = IO::template load<q>(v[i]);
     });
 
     // the loads above are issued before this test (every row they touch lies
     // inside the ghosted box), so the kind lookup costs no extra round trip
-    if (!any) return;
+    if (!any) return true;
 
     Calc alpha[V];
     if constexpr (BRINK) {
@@ -218,16 +266,18 @@ __device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int
         for (int i = 0; i < V; ++i) alpha[i] = act[i] ? al[e0 + i] : Calc(0);
     }
 
+    bool finite = true;
 #pragma unroll
     for (int i = 0; i < V; ++i) {
         if constexpr (M == Mode::EXACT64) {
             collide_exact<L>(f[i], a.tau);
         } else {
-            collide_shifted<L, Calc, BRINK, kRcp<M>>(f[i], static_cast<Calc>(a.omega),
+            const Calc base = collide_shifted<L, Calc, BRINK, kRcp<M>, IO::SX>(f[i], static_cast<Calc>(a.omega),
                                             BRINK ? alpha[i] : Calc(0),
                                             static_cast<Calc>(a.tau * a.g[0]),
                                             static_cast<Calc>(a.tau * a.g[1]),
                                             static_cast<Calc>(a.tau * a.g[2]));
+            finite &= !act[i] || finite_calc(base);
         }
     }
 
@@ -240,14 +290,15 @@ __device__ __forceinline__ void bulk_cells(const StepArgs& a, int x0, int y, int
         if (all) {
             VecT<T, V> o;
 #pragma unroll
-            for (int i = 0; i < V; ++i) o.v[i] = IO::template from_calc<q>(f[i][q]);
+            for (int i = 0; i < V; ++i) o.v[i] = IO::template store<q>(f[i][q]);
             store_stream<T, V>(out, o);
         } else {
 #pragma unroll
             for (int i = 0; i < V; ++i)
-                if (act[i]) store_stream1<T>(out + i, IO::template from_calc<q>(f[i][q]));
+                if (act[i]) store_stream1<T>(out + i, IO::template store<q>(f[i][q]));
         }
     });
+    return finite;
 }
 
 template <class L, Mode M, int V, bool FLAGS, bool BRINK>
@@ -317,14 +368,15 @@ __device__ __forceinline__ OwnSums<typename Io<M, L>::Calc> own_sums_at(
 
 // Value of a NoSlip, MovingWall, Inflow, Outflow or Pressure rule for
 // direction q, from x's own sums and own[qbar] (finv).
-template <class L, Mode M, int q>
-__device__ __forceinline__ typename Io<M, L>::Calc rule_value(const Cond& cd,
+template <class L, Mode M, int q, int RULES = RULES_ALL>
+__device__ __forceinline__ typename Io<M, L>::Calc rule_value(const GCond& cd,
                                                              const OwnSums<typename Io<M, L>::Calc>& os,
                                                              typename Io<M, L>::Calc finv) {
     using Calc = typename Io<M, L>::Calc;
     constexpr int Q = L::Q;
-    switch (cd.kind) {
-        case BC_MOVINGWALL:  // boundary.hpp:55-65
+    const int kind = cd.kind;
+    if constexpr ((RULES & RULE_MOVINGWALL) != 0) {
+        if (kind == BC_MOVINGWALL) {  // boundary.hpp:55-65
             if constexpr (M == Mode::EXACT64) {
                 double c_dot_u = 0;
                 c_dot_u += static_cast<double>(L::cx(q)) * cd.vel[0];
@@ -334,14 +386,21 @@ __device__ __forceinline__ typename Io<M, L>::Calc rule_value(const Cond& cd,
                 return finv + 2.0 * weight<L>(q) * os.s * c_dot_u / (cs * cs);
             } else {
                 const Calc rho = Calc(1) + os.s;
-                const Calc cu = static_cast<Calc>(L::cx(q) * cd.vel[0] + L::cy(q) * cd.vel[1] +
-                                                  L::cz(q) * cd.vel[2]);
+                Calc cu;
+                if constexpr (std::is_same<Calc, float>::value)
+                    cu = cd.cu32[q];
+                else
+                    cu = static_cast<Calc>(cd.cu64[q]);
                 return finv + static_cast<Calc>(6.0 * weight<L>(q)) * rho * cu;
             }
-        case BC_INFLOW:  // boundary.hpp:176-178, feq precomputed on the host
+        }
+    }
+    if constexpr ((RULES & RULE_INFLOW) != 0) {
+        if (kind == BC_INFLOW)  // boundary.hpp:176-178, feq precomputed on the host
             return (M == Mode::EXACT64) ? static_cast<Calc>(cd.feq[q]) : static_cast<Calc>(cd.geq[q]);
-        case BC_OUTFLOW:     // boundary.hpp:140-147
-        case BC_PRESSURE:    // boundary.hpp:206-211
+    }
+    if constexpr ((RULES & RULE_OUTFLOW) != 0) {
+        if (kind == BC_OUTFLOW || kind == BC_PRESSURE) {  // boundary.hpp:140-147, 206-211
             if constexpr (M == Mode::EXACT64) {
                 const double u[3] = {os.m0 / cd.rho, os.m1 / cd.rho, os.m2 / cd.rho};
                 const double fq = feq_exact<L, q>(cd.rho, u);
@@ -357,9 +416,9 @@ __device__ __forceinline__ typename Io<M, L>::Calc rule_value(const Cond& cd,
                 const Calc even = (rr - Calc(1)) + rr * (Calc(4.5) * cu * cu - Calc(1.5) * uu);
                 return static_cast<Calc>(2.0 * weight<L>(q)) * even - finv;
             }
-        default:  // BC_NOSLIP, boundary.hpp:26-28
-            return finv;
+        }
     }
+    return finv;  // BC_NOSLIP, boundary.hpp:26-28
 }
 
 // FreeSlip candidates (include/boundary.hpp:93-110), in order: which of the
@@ -388,7 +447,7 @@ constexpr int dir_of(int cx, int cy, int cz) {
 // (planes under 4 GiB).
 template <class L, Mode M, int q, bool FS = true, bool B32 = false, class OwnFn>
 __device__ __forceinline__ typename Io<M, L>::Calc pull_boundary(
-    const StepArgs& a, const Cond& cd, const OwnSums<typename Io<M, L>::Calc>& os, OwnFn own, int x,
+    const StepArgs& a, const GCond& cd, const OwnSums<typename Io<M, L>::Calc>& os, OwnFn own, int x,
     int y, int z, int sx, int sy, int sz, typename ElemIdx<B32>::type se, bool& stale) {
     using IO = Io<M, L>;
     using T = typename IO::T;
@@ -412,7 +471,7 @@ __device__ __forceinline__ typename Io<M, L>::Calc pull_boundary(
                 if (!found) {
                     const auto ce = elem_at<B32>(Ly, kFsSwap[j][0] ? x : sx, kFsSwap[j][1] ? y : sy,
                                                  kFsSwap[j][2] ? z : sz);
-                    if (a.kind[ce] < KIND_COND0) {
+                    if (kind_at(a.kind, ce) < KIND_COND0) {
                         v = IO::template to_calc<rq>(plane_ref<B32>(src, rq, Ly.plane, ce));
                         found = true;
                     }
@@ -433,7 +492,7 @@ __device__ __forceinline__ typename Io<M, L>::Calc pull_boundary(
 // The same with x's own values already in registers.
 template <class L, Mode M, int q, bool FS = true>
 __device__ __forceinline__ typename Io<M, L>::Calc pull_nonfluid(
-    const StepArgs& a, const Cond& cd, const typename Io<M, L>::Calc (&own)[L::Q], int x, int y,
+    const StepArgs& a, const GCond& cd, const typename Io<M, L>::Calc (&own)[L::Q], int x, int y,
     int z, int sx, int sy, int sz, int64_t se, bool& stale) {
     using Calc = typename Io<M, L>::Calc;
     const OwnSums<Calc> os = own_sums<L, Calc>(own);
@@ -442,9 +501,10 @@ __device__ __forceinline__ typename Io<M, L>::Calc pull_nonfluid(
 }
 
 // ---------------------------------------------------------------------------
-// One NEAR cell (fluid with a non-fluid source).
+// One NEAR cell (fluid with a non-fluid source). Returns false when the cell
+// read a NaN or infinite value.
 template <class L, Mode M>
-__device__ __forceinline__ void near_cell(const StepArgs& a, int x, int y, int z) {
+__device__ __forceinline__ bool near_cell(const StepArgs& a, int x, int y, int z) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
@@ -470,35 +530,37 @@ __device__ __forceinline__ void near_cell(const StepArgs& a, int x, int y, int z
         const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
         const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
         const int64_t se = cell_elem(Ly, sx, sy, sz);
-        const uint8_t k = a.kind[se];
+        const uint8_t k = kind_at(a.kind, se);
         if (k < KIND_COND0 || a.materialised) {
-            f[q] = IO::template to_calc<q>(src[q * Ly.plane + se]);
+            f[q] = IO::template load<q>(src[q * Ly.plane + se]);
         } else {
             bool stale;
-            f[q] = pull_boundary<L, M, q>(a, a.conds[k - KIND_COND0], os, own, x, y, z, sx, sy, sz, se,
-                                          stale);
+            f[q] = IO::template scaled<q>(pull_boundary<L, M, q>(a, cond_at(a.conds, k - KIND_COND0), os, own, x, y,
+                                                                 z, sx, sy, sz, se, stale));
         }
     });
 
+    bool finite = true;
     if constexpr (M == Mode::EXACT64) {
         collide_exact<L>(f, a.tau);
     } else {
         Calc alpha = Calc(0);
         if (a.alpha) alpha = static_cast<const Calc*>(a.alpha)[e];
         if (a.alpha) {
-            collide_shifted<L, Calc, true, kRcp<M>>(f, static_cast<Calc>(a.omega), alpha,
+            finite = finite_calc(collide_shifted<L, Calc, true, kRcp<M>, IO::SX>(f, static_cast<Calc>(a.omega), alpha,
                                            static_cast<Calc>(a.tau * a.g[0]),
                                            static_cast<Calc>(a.tau * a.g[1]),
-                                           static_cast<Calc>(a.tau * a.g[2]));
+                                           static_cast<Calc>(a.tau * a.g[2])));
         } else {
-            collide_shifted<L, Calc, false, kRcp<M>>(f, static_cast<Calc>(a.omega), Calc(0), Calc(0),
-                                            Calc(0), Calc(0));
+            finite = finite_calc(collide_shifted<L, Calc, false, kRcp<M>, IO::SX>(f, static_cast<Calc>(a.omega), Calc(0),
+                                                                          Calc(0), Calc(0), Calc(0)));
         }
     }
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        dst[q * Ly.plane + e] = IO::template from_calc<q>(f[q]);
+        dst[q * Ly.plane + e] = IO::template store<q>(f[q]);
     });
+    return finite;
 }
 
 // ---------------------------------------------------------------------------
@@ -510,9 +572,9 @@ __device__ __forceinline__ void near_cell(const StepArgs& a, int x, int y, int z
 // values (as near_cell), and all lanes collide and store together. The bits
 // equal those of bulk_cells / near_cell for the same cell. B32 takes the
 // cell's own and plain-pull addresses as 32-bit byte offsets (planes under
-// 4 GiB).
+// 4 GiB). Returns false when a fluid cell read a NaN or infinite value.
 template <class L, Mode M, bool FS = true, bool B32 = false>
-__device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z) {
+__device__ __forceinline__ bool any_cell(const StepArgs& a, int x, int y, int z) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
@@ -520,7 +582,7 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
     constexpr int Q = L::Q;
     const FieldLayout& Ly = a.L;
     const I e = elem_at<B32>(Ly, x, y, z);
-    const uint8_t k = a.kind[e];
+    const uint8_t k = kind_at(a.kind, e);
     const T* __restrict__ src = static_cast<const T*>(a.src);
     T* __restrict__ dst = static_cast<T*>(a.dst);
 
@@ -538,7 +600,7 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
             constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
             const int64_t d = cx + (cy > 0 ? Ly.px : (cy < 0 ? -Ly.px : 0)) +
                               (cz > 0 ? Ly.pxy : (cz < 0 ? -Ly.pxy : 0));
-            f[q] = IO::template to_calc<q>(plane_ref<B32>(src, q, Ly.plane, e - static_cast<I>(d)));
+            f[q] = IO::template load<q>(plane_ref<B32>(src, q, Ly.plane, e - static_cast<I>(d)));
         });
     } else {
         static_for<0, Q>([&](auto qc) {
@@ -547,48 +609,51 @@ __device__ __forceinline__ void any_cell(const StepArgs& a, int x, int y, int z)
             const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
             const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
             const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
-            f[q] = IO::template to_calc<q>(plane_ref<B32>(src, q, Ly.plane, elem_at<B32>(Ly, sx, sy, sz)));
+            f[q] = IO::template load<q>(plane_ref<B32>(src, q, Ly.plane, elem_at<B32>(Ly, sx, sy, sz)));
         });
     }
-    if (k >= KIND_COND0) return;
+    if (k >= KIND_COND0) return true;
     if (k == KIND_NEAR && !a.materialised) {
         const OwnSums<Calc> os = own_sums_at<L, M, B32>(src, Ly.plane, e);
         auto own = [&](auto kc) {
             constexpr int kk = decltype(kc)::value;
             return IO::template to_calc<kk>(plane_ref<B32>(src, kk, Ly.plane, e));
         };
+        const uint32_t lm = glob(a.lmask)[e];
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
-            const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
-            const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
-            const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
-            const I se = elem_at<B32>(Ly, sx, sy, sz);
-            const uint8_t sk = a.kind[se];
-            if (sk >= KIND_COND0) {
+            if (lm & (1u << q)) {
+                const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
+                const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
+                const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
+                const I se = elem_at<B32>(Ly, sx, sy, sz);
+                const uint8_t sk = kind_at(a.kind, se);
                 bool stale;
-                f[q] = pull_boundary<L, M, q, FS, B32>(a, a.conds[sk - KIND_COND0], os, own, x, y, z, sx, sy,
-                                                       sz, se, stale);
+                f[q] = IO::template scaled<q>(pull_boundary<L, M, q, FS, B32>(a, cond_at(a.conds, sk - KIND_COND0), os,
+                                                                               own, x, y, z, sx, sy, sz, se, stale));
             }
         });
     }
 
+    bool finite = true;
     if constexpr (M == Mode::EXACT64) {
         collide_exact<L>(f, a.tau);
     } else if (a.alpha) {
         const Calc alpha = static_cast<const Calc*>(a.alpha)[e];
-        collide_shifted<L, Calc, true, kRcp<M>>(f, static_cast<Calc>(a.omega), alpha,
+        finite = finite_calc(collide_shifted<L, Calc, true, kRcp<M>, IO::SX>(f, static_cast<Calc>(a.omega), alpha,
                                                 static_cast<Calc>(a.tau * a.g[0]),
                                                 static_cast<Calc>(a.tau * a.g[1]),
-                                                static_cast<Calc>(a.tau * a.g[2]));
+                                                static_cast<Calc>(a.tau * a.g[2])));
     } else {
-        collide_shifted<L, Calc, false, kRcp<M>>(f, static_cast<Calc>(a.omega), Calc(0), Calc(0),
-                                                 Calc(0), Calc(0));
+        finite = finite_calc(collide_shifted<L, Calc, false, kRcp<M>, IO::SX>(f, static_cast<Calc>(a.omega), Calc(0),
+                                                                      Calc(0), Calc(0), Calc(0)));
     }
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        store_stream1<T>(&plane_ref<B32>(dst, q, Ly.plane, e), IO::template from_calc<q>(f[q]));
+        store_stream1<T>(&plane_ref<B32>(dst, q, Ly.plane, e), IO::template store<q>(f[q]));
     });
+    return finite;
 }
 
 // Cell-tile kernel: wavefront w of the grid takes the 64 cells

@@ -78,21 +78,24 @@ __device__ __forceinline__ void aa_collide(const StepArgs& a, int64_t e, typenam
         collide_exact<L>(f, a.tau);
     } else if (a.alpha) {
         const Calc alpha = static_cast<const Calc*>(a.alpha)[e];
-        collide_shifted<L, Calc, true, kRcp<M>>(f, static_cast<Calc>(a.omega), alpha, static_cast<Calc>(a.tau * a.g[0]),
+        collide_shifted<L, Calc, true, kRcp<M>, Io<M, L>::SX>(f, static_cast<Calc>(a.omega), alpha,
+                                                              static_cast<Calc>(a.tau * a.g[0]),
                                                 static_cast<Calc>(a.tau * a.g[1]), static_cast<Calc>(a.tau * a.g[2]));
     } else {
-        collide_shifted<L, Calc, false, kRcp<M>>(f, static_cast<Calc>(a.omega), Calc(0), Calc(0), Calc(0), Calc(0));
+        collide_shifted<L, Calc, false, kRcp<M>, Io<M, L>::SX>(f, static_cast<Calc>(a.omega), Calc(0), Calc(0), Calc(0),
+                                                               Calc(0));
     }
 }
 
-// x's post-collision values as they would be stored (rounded to the storage
-// format), which is what the two-buffer kernels' rules read back.
+// x's post-collision values (f: the collision's scaled representation) as
+// they would be stored (rounded to the storage format), which is what the
+// two-buffer kernels' rules read back.
 template <class L, Mode M>
 __device__ __forceinline__ void aa_own(const typename Io<M, L>::Calc (&f)[L::Q], typename Io<M, L>::Calc (&own)[L::Q]) {
     using IO = Io<M, L>;
     static_for<0, L::Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        own[q] = IO::template to_calc<q>(IO::template from_calc<q>(f[q]));
+        own[q] = IO::template to_calc<q>(IO::template store<q>(f[q]));
     });
 }
 
@@ -107,19 +110,19 @@ __global__ void __launch_bounds__(256) k_aa_eo(StepArgs a, int tpp, double rxl) 
     if (!aa_tile_cell(Ly, tpp, rxl, x, y, z)) return;
     using I = typename ElemIdx<B32>::type;
     const I e = static_cast<I>(cell_elem(Ly, x, y, z));
-    const uint8_t k = a.kind[e];
+    const uint8_t k = kind_at(a.kind, e);
     if (k >= KIND_COND0) return;
     T* __restrict__ m = static_cast<T*>(a.dst);
     const bool wraps = aa_wraps(Ly, x, y, z);
     Calc f[Q];
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        f[q] = IO::template to_calc<q>(plane_ref<B32>(m, q, Ly.plane, aa_nb<L, q>(Ly, wraps, e, x, y, z, -1)));
+        f[q] = IO::template load<q>(plane_ref<B32>(m, q, Ly.plane, aa_nb<L, q>(Ly, wraps, e, x, y, z, -1)));
     });
     aa_collide<L, M>(a, static_cast<int64_t>(e), f);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        plane_ref<B32>(m, Q - 1 - q, Ly.plane, aa_nb<L, q>(Ly, wraps, e, x, y, z, 1)) = IO::template from_calc<q>(f[q]);
+        plane_ref<B32>(m, Q - 1 - q, Ly.plane, aa_nb<L, q>(Ly, wraps, e, x, y, z, 1)) = IO::template store<q>(f[q]);
     });
     if (k == KIND_NEAR) {
         Calc own[Q];
@@ -128,10 +131,10 @@ __global__ void __launch_bounds__(256) k_aa_eo(StepArgs a, int tpp, double rxl) 
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int j = Q - 1 - q;  // the link of x whose source is x + c_q
-            const uint8_t kb = a.kind[aa_nb<L, q>(Ly, wraps, e, x, y, z, 1)];
+            const uint8_t kb = kind_at(a.kind, aa_nb<L, q>(Ly, wraps, e, x, y, z, 1));
             if (kb >= KIND_COND0 && kb < KIND_INERT)
                 plane_ref<B32>(m, q, Ly.plane, e) =
-                    IO::template from_calc<j>(rule_value<L, M, j>(a.conds[kb - KIND_COND0], os, own[q]));
+                    IO::template from_calc<j>(rule_value<L, M, j>(cond_at(a.conds, kb - KIND_COND0), os, own[q]));
         });
     }
 }
@@ -147,18 +150,18 @@ __global__ void __launch_bounds__(256) k_aa_oe(StepArgs a, int tpp, double rxl) 
     if (!aa_tile_cell(Ly, tpp, rxl, x, y, z)) return;
     using I = typename ElemIdx<B32>::type;
     const I e = static_cast<I>(cell_elem(Ly, x, y, z));
-    const uint8_t k = a.kind[e];
+    const uint8_t k = kind_at(a.kind, e);
     if (k >= KIND_COND0) return;
     T* __restrict__ m = static_cast<T*>(a.dst);
     Calc f[Q];
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        f[q] = IO::template to_calc<q>(plane_ref<B32>(m, Q - 1 - q, Ly.plane, e));
+        f[q] = IO::template load<q>(plane_ref<B32>(m, Q - 1 - q, Ly.plane, e));
     });
     aa_collide<L, M>(a, static_cast<int64_t>(e), f);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        plane_ref<B32>(m, q, Ly.plane, e) = IO::template from_calc<q>(f[q]);
+        plane_ref<B32>(m, q, Ly.plane, e) = IO::template store<q>(f[q]);
     });
     if (k == KIND_NEAR) {
         const bool wraps = aa_wraps(Ly, x, y, z);
@@ -168,10 +171,10 @@ __global__ void __launch_bounds__(256) k_aa_oe(StepArgs a, int tpp, double rxl) 
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             const I be = aa_nb<L, q>(Ly, wraps, e, x, y, z, -1);
-            const uint8_t kb = a.kind[be];
+            const uint8_t kb = kind_at(a.kind, be);
             if (kb >= KIND_COND0 && kb < KIND_INERT)
                 plane_ref<B32>(m, q, Ly.plane, be) =
-                    IO::template from_calc<q>(rule_value<L, M, q>(a.conds[kb - KIND_COND0], os, own[Q - 1 - q]));
+                    IO::template from_calc<q>(rule_value<L, M, q>(cond_at(a.conds, kb - KIND_COND0), os, own[Q - 1 - q]));
         });
     }
 }
@@ -188,16 +191,16 @@ __global__ void __launch_bounds__(256) k_aa_swap(StepArgs a, int tpp, double rxl
     if (!aa_tile_cell(Ly, tpp, rxl, x, y, z)) return;
     using I = typename ElemIdx<B32>::type;
     const I e = static_cast<I>(cell_elem(Ly, x, y, z));
-    if (a.kind[e] >= KIND_COND0) return;
+    if (kind_at(a.kind, e) >= KIND_COND0) return;
     T* __restrict__ m = static_cast<T*>(a.dst);
     const bool wraps = aa_wraps(Ly, x, y, z);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (q != L::rest) {
             const I pe = aa_nb<L, q>(Ly, wraps, e, x, y, z, 1);
-            if (q < Q / 2 || a.kind[pe] >= KIND_COND0) {
-                T* p0 = &plane_ref<B32>(m, q, Ly.plane, e);
-                T* p1 = &plane_ref<B32>(m, Q - 1 - q, Ly.plane, pe);
+            if (q < Q / 2 || kind_at(a.kind, pe) >= KIND_COND0) {
+                Glob<T>* p0 = &plane_ref<B32>(m, q, Ly.plane, e);
+                Glob<T>* p1 = &plane_ref<B32>(m, Q - 1 - q, Ly.plane, pe);
                 const T v0 = *p0;
                 *p0 = *p1;
                 *p1 = v0;

@@ -10,7 +10,15 @@
 //                                            (the adjoint of bounce-back),
 //                 + sum_{moving-wall links q of x} mu^{t+1}_q(x) 6 w_q c_q.u_w
 //                                            (the wall rule's density coupling),
-//                 = 0                        if x + c_k is an inflow cell.
+//                 = 0                        if x + c_k is an inflow cell,
+//                 = -mu^{t+1}_kbar(x) + c_k.W  if x + c_k is an outflow or
+//                                            pressure cell (below).
+// An outflow or pressure link j of x (x - c_j such a cell, reference density
+// r) pulls h_j = 2 w_j ((r - 1) + r (4.5 (c_j.u)^2 - 1.5 u.u)) - g_jbar(x)
+// with u = m / r, m the momentum of x's own g^t (include/boundary.hpp:
+// 140-147, 206-211). Its derivative by g_k(x) is
+// -delta(k, jbar) + 2 w_j (9 (c_j.u)(c_j.c_k) - 3 u.c_k), so every g_k(x)
+// also receives c_k.W with W = sum_j mu_j(x) 2 w_j (9 (c_j.u_j) c_j - 3 u_j).
 // The kernels therefore keep mu (not lambda) in memory: each backward step
 // reads g^{t-1} (pull), mu^{t+1} (reverse pull) and writes mu^t, and adds
 // dJ/dalpha(x) and dJ/du_wall of step t into fp64 accumulators.
@@ -193,14 +201,14 @@ __global__ void __launch_bounds__(256) k_adj_bulk(AdjArgs a, int nbx, int nby) {
     const int y = 1 + by * blockDim.y + threadIdx.y;
     if (x > Ly.xl || y > Ly.yl) return;
     const int64_t e = cell_elem(Ly, x, y, z);
-    if (a.kind[e] != KIND_BULK) return;
+    if (kind_at(a.kind, e) != KIND_BULK) return;
     adj_bulk_cell<L, M, BRINK>(a, x, y, z, e);
 }
 
 // Backward step of one fluid cell of any kind, with the adjoint boundary
-// rules of NoSlip, MovingWall and Inflow sources (other kinds are rejected on
-// the host). gw (3 doubles) accumulates dJ/du_wall of this cell's moving-wall
-// links.
+// rules of NoSlip, MovingWall, Inflow, Outflow and Pressure sources (FreeSlip
+// is rejected on the host). gw (3 doubles) accumulates dJ/du_wall of this
+// cell's moving-wall links.
 template <class L, Mode M, bool BRINK>
 __device__ __forceinline__ void adj_cell(const AdjArgs& a, int x, int y, int z, double* __restrict__ gw) {
     using IO = Io<M, L>;
@@ -232,13 +240,13 @@ __device__ __forceinline__ void adj_cell(const AdjArgs& a, int x, int y, int z, 
         const int sy = wrap_if(y - L::cy(q), Ly.yl, Ly.per[1]);
         const int sz = wrap_if(z - L::cz(q), Ly.zl, Ly.per[2]);
         const int64_t se = cell_elem(Ly, sx, sy, sz);
-        const uint8_t k = a.kind[se];
+        const uint8_t k = kind_at(a.kind, se);
         if (k < KIND_COND0 || a.materialised) {
             h[q] = IO::template to_calc<q>(gp[q * Ly.plane + se]);
         } else {
             bool stale;
             // (the adjoint takes no FreeSlip condition)
-            h[q] = pull_nonfluid<L, M, q, false>(sa, a.conds[k - KIND_COND0], own, x, y, z, sx, sy, sz, se, stale);
+            h[q] = pull_nonfluid<L, M, q, false>(sa, cond_at(a.conds, k - KIND_COND0), own, x, y, z, sx, sy, sz, se, stale);
         }
     });
 
@@ -253,6 +261,7 @@ __device__ __forceinline__ void adj_cell(const AdjArgs& a, int x, int y, int z, 
         objective_grad<L, Calc>(a, x, y, z, gf, lam);
     } else {
         Calc wall_term = Calc(0);
+        uint32_t outlinks = 0;  // bit q: x pulls q from an outflow or pressure cell
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             // consumer of g_q(x) in step t+1: the cell x + c_q, or x itself via a wall
@@ -260,28 +269,67 @@ __device__ __forceinline__ void adj_cell(const AdjArgs& a, int x, int y, int z, 
             const int ny = wrap_if(y + L::cy(q), Ly.yl, Ly.per[1]);
             const int nz = wrap_if(z + L::cz(q), Ly.zl, Ly.per[2]);
             const int64_t ne = cell_elem(Ly, nx, ny, nz);
-            const uint8_t k = a.kind[ne];
+            const uint8_t k = kind_at(a.kind, ne);
             if (k < KIND_COND0) {
                 lam[q] = mi[q * Ly.plane + ne];
             } else {
-                const int kind = a.conds[k - KIND_COND0].kind;
-                lam[q] = (kind == BC_NOSLIP || kind == BC_MOVINGWALL) ? mi[(Q - 1 - q) * Ly.plane + e]
-                                                                      : Calc(0);
+                const int kind = cond_at(a.conds, k - KIND_COND0).kind;
+                const Calc back = mi[(Q - 1 - q) * Ly.plane + e];
+                lam[q] = (kind == BC_NOSLIP || kind == BC_MOVINGWALL)
+                             ? back
+                             : ((kind == BC_OUTFLOW || kind == BC_PRESSURE) ? -back : Calc(0));
             }
             // x pulls direction q from the wall at x - c_q: density coupling
             const int sx = wrap_if(x - L::cx(q), Ly.xl, Ly.per[0]);
             const int sy = wrap_if(y - L::cy(q), Ly.yl, Ly.per[1]);
             const int sz = wrap_if(z - L::cz(q), Ly.zl, Ly.per[2]);
-            const uint8_t ks = a.kind[cell_elem(Ly, sx, sy, sz)];
+            const uint8_t ks = kind_at(a.kind, cell_elem(Ly, sx, sy, sz));
             if (ks >= KIND_COND0 && ks < KIND_INERT) {
-                const Cond& cd = a.conds[ks - KIND_COND0];
+                const GCond& cd = cond_at(a.conds, ks - KIND_COND0);
                 if (cd.kind == BC_MOVINGWALL) {
                     const double cu = L::cx(q) * cd.vel[0] + L::cy(q) * cd.vel[1] + L::cz(q) * cd.vel[2];
                     wall_term += mi[q * Ly.plane + e] * static_cast<Calc>(6.0 * weight<L>(q) * cu);
                 }
+                if (cd.kind == BC_OUTFLOW || cd.kind == BC_PRESSURE) outlinks |= 1u << q;
             }
         });
-        static_for<0, Q>([&](auto qc) { lam[decltype(qc)::value] += wall_term; });
+        Calc Wx = Calc(0), Wy = Calc(0), Wz = Calc(0);
+        if (outlinks) {
+            // momentum of x's own g^t: the forward collision of h, as stored
+            Calc gt[Q];
+            static_for<0, Q>([&](auto qc) { gt[decltype(qc)::value] = h[decltype(qc)::value]; });
+            Calc al = Calc(0);
+            if constexpr (BRINK) al = static_cast<const Calc*>(a.alpha)[e];
+            collide_shifted<L, Calc, BRINK, kRcp<M>, IO::SX>(gt, static_cast<Calc>(a.omega), al,
+                                                            static_cast<Calc>(a.tau * a.g[0]),
+                                                            static_cast<Calc>(a.tau * a.g[1]),
+                                                            static_cast<Calc>(a.tau * a.g[2]));
+            OwnAcc<L, Calc> acc;
+            static_for<0, Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                acc.template add<q>(IO::template to_calc<q>(IO::template store<q>(gt[q])));
+            });
+            static_for<0, Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if (outlinks & (1u << q)) {
+                    const int sx = wrap_if(x - L::cx(q), Ly.xl, Ly.per[0]);
+                    const int sy = wrap_if(y - L::cy(q), Ly.yl, Ly.per[1]);
+                    const int sz = wrap_if(z - L::cz(q), Ly.zl, Ly.per[2]);
+                    const GCond& cd = cond_at(a.conds, kind_at(a.kind, cell_elem(Ly, sx, sy, sz)) - KIND_COND0);
+                    const Calc ir = Calc(1) / static_cast<Calc>(cd.rho);
+                    const Calc ux = acc.r.m0 * ir, uy = acc.r.m1 * ir, uz = acc.r.m2 * ir;
+                    const Calc cu = Calc(L::cx(q)) * ux + Calc(L::cy(q)) * uy + Calc(L::cz(q)) * uz;
+                    const Calc f = mi[q * Ly.plane + e] * static_cast<Calc>(2.0 * weight<L>(q));
+                    Wx += f * (Calc(9) * cu * Calc(L::cx(q)) - Calc(3) * ux);
+                    Wy += f * (Calc(9) * cu * Calc(L::cy(q)) - Calc(3) * uy);
+                    Wz += f * (Calc(9) * cu * Calc(L::cz(q)) - Calc(3) * uz);
+                }
+            });
+        }
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            lam[q] += wall_term + (Calc(L::cx(q)) * Wx + Calc(L::cy(q)) * Wy + Calc(L::cz(q)) * Wz);
+        });
     }
 
     Calc alpha = Calc(0);
@@ -306,8 +354,8 @@ __device__ __forceinline__ void adj_cell(const AdjArgs& a, int x, int y, int z, 
             const int sx = wrap_if(x - L::cx(q), Ly.xl, Ly.per[0]);
             const int sy = wrap_if(y - L::cy(q), Ly.yl, Ly.per[1]);
             const int sz = wrap_if(z - L::cz(q), Ly.zl, Ly.per[2]);
-            const uint8_t ks = a.kind[cell_elem(Ly, sx, sy, sz)];
-            if (ks >= KIND_COND0 && ks < KIND_INERT && a.conds[ks - KIND_COND0].kind == BC_MOVINGWALL) {
+            const uint8_t ks = kind_at(a.kind, cell_elem(Ly, sx, sy, sz));
+            if (ks >= KIND_COND0 && ks < KIND_INERT && cond_at(a.conds, ks - KIND_COND0).kind == BC_MOVINGWALL) {
                 const double f = static_cast<double>(mu[q] * rho_own) * 6.0 * weight<L>(q);
                 gw0 += f * L::cx(q);
                 gw1 += f * L::cy(q);
@@ -316,9 +364,10 @@ __device__ __forceinline__ void adj_cell(const AdjArgs& a, int x, int y, int z, 
             }
         });
         if (walls) {
-            gw[0] += gw0;
-            gw[1] += gw1;
-            gw[2] += gw2;
+            Glob<double>* g = glob(gw);
+            g[0] += gw0;
+            g[1] += gw1;
+            g[2] += gw2;
         }
     }
 }
@@ -375,10 +424,10 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
             const int sy = wrap_if(y - L::cy(q), Ly.yl, Ly.per[1]);
             const int sz = wrap_if(z - L::cz(q), Ly.zl, Ly.per[2]);
             const I se = elem_at<B32>(Ly, sx, sy, sz);
-            const uint8_t k = a.kind[se];
+            const uint8_t k = kind_at(a.kind, se);
             if (k >= KIND_COND0) {
                 bool stale;
-                h[q] = pull_boundary<L, M, q, false, B32>(sa, a.conds[k - KIND_COND0], os, own, x, y, z, sx, sy, sz,
+                h[q] = pull_boundary<L, M, q, false, B32>(sa, cond_at(a.conds, k - KIND_COND0), os, own, x, y, z, sx, sy, sz,
                                                           se, stale);
             }
         });
@@ -402,15 +451,15 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
             Calc wall_term = Calc(0);
             static_for<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                const uint8_t k = a.kind[nb_elem(qc, 1)];
+                const uint8_t k = kind_at(a.kind, nb_elem(qc, 1));
                 if (k >= KIND_COND0) {
-                    const int kind = a.conds[k - KIND_COND0].kind;
+                    const int kind = cond_at(a.conds, k - KIND_COND0).kind;
                     lam[q] = (kind == BC_NOSLIP || kind == BC_MOVINGWALL) ? plane_ref<B32>(mi, Q - 1 - q, Ly.plane, e)
                                                                           : Calc(0);
                 }
-                const uint8_t ks = a.kind[nb_elem(qc, -1)];
+                const uint8_t ks = kind_at(a.kind, nb_elem(qc, -1));
                 if (ks >= KIND_COND0 && ks < KIND_INERT) {
-                    const Cond& cd = a.conds[ks - KIND_COND0];
+                    const GCond& cd = cond_at(a.conds, ks - KIND_COND0);
                     if (cd.kind == BC_MOVINGWALL) {
                         const double cu = L::cx(q) * cd.vel[0] + L::cy(q) * cd.vel[1] + L::cz(q) * cd.vel[2];
                         wall_term += plane_ref<B32>(mi, q, Ly.plane, e) * static_cast<Calc>(6.0 * weight<L>(q) * cu);
@@ -444,8 +493,8 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
         double gw0 = 0, gw1 = 0, gw2 = 0;
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const uint8_t ks = a.kind[nb_elem(qc, -1)];
-            if (ks >= KIND_COND0 && ks < KIND_INERT && a.conds[ks - KIND_COND0].kind == BC_MOVINGWALL) {
+            const uint8_t ks = kind_at(a.kind, nb_elem(qc, -1));
+            if (ks >= KIND_COND0 && ks < KIND_INERT && cond_at(a.conds, ks - KIND_COND0).kind == BC_MOVINGWALL) {
                 const double f = static_cast<double>(mu[q] * rho_own) * 6.0 * weight<L>(q);
                 gw0 += f * L::cx(q);
                 gw1 += f * L::cy(q);
@@ -454,9 +503,10 @@ __device__ __forceinline__ void adj_any_cell(const AdjArgs& a, int x, int y, int
             }
         });
         if (walls) {
-            gw[0] += gw0;
-            gw[1] += gw1;
-            gw[2] += gw2;
+            Glob<double>* g = glob(gw);
+            g[0] += gw0;
+            g[1] += gw1;
+            g[2] += gw2;
         }
     }
 }

@@ -226,7 +226,7 @@ __global__ void k_materialise(StepArgs a, GhostDims g) {
     const int64_t be = cell_elem(Ly, bx, by, bz);
     const uint8_t k = a.kind[be];
     if (k < KIND_COND0 || k >= KIND_INERT) return;
-    const Cond& cd = a.conds[k - KIND_COND0];
+    const GCond& cd = cond_at(a.conds, k - KIND_COND0);
     const T* src = static_cast<const T*>(a.src);
     T* dst = static_cast<T*>(a.dst);
     static_for<0, Q>([&](auto qc) {

@@ -82,7 +82,7 @@ template <class L, Mode M, bool BRINK>
 __device__ __forceinline__ void tb_collide(const StepArgs& a, typename Io<M, L>::Calc (&f)[L::Q],
                                            typename Io<M, L>::Calc alpha) {
     using Calc = typename Io<M, L>::Calc;
-    collide_shifted<L, Calc, BRINK, kRcp<M>>(f, static_cast<Calc>(a.omega), BRINK ? alpha : Calc(0),
+    collide_shifted<L, Calc, BRINK, kRcp<M>, Io<M, L>::SX>(f, static_cast<Calc>(a.omega), BRINK ? alpha : Calc(0),
                                              static_cast<Calc>(a.tau * a.g[0]), static_cast<Calc>(a.tau * a.g[1]),
                                              static_cast<Calc>(a.tau * a.g[2]));
 }
@@ -166,14 +166,14 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                 // source x - c_q: index 0 for c = +1, 1 for 0, 2 for -1
                 constexpr int ix = 1 - L::cx(q), iy = 1 - L::cy(q), iz = 1 - L::cz(q);
                 const uint32_t o = zs3[iz] + ys[iy] + xs[ix];
-                f[q] = IO::template to_calc<q>(tb_at(src + q * plane, o));
+                f[q] = IO::template load<q>(tb_at(src + q * plane, o));
             });
             Calc al = Calc(0);
             if constexpr (BRINK) al = alpha[(zs3[1] + ys[1] + xs[1]) / TS];
             tb_collide<L, M, BRINK>(a, f, al);
             static_for<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                const T v = IO::template from_calc<q>(f[q]);
+                const T v = IO::template store<q>(f[q]);
                 if constexpr (L::cz(q) < 0) {
                     const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
                     if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY)
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                     Calc f[Q];
                     static_for<0, Q>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
-                        f[q] = IO::template to_calc<q>(lds[S::template slot<q>()][c]);
+                        f[q] = IO::template load<q>(lds[S::template slot<q>()][c]);
                     });
                     const uint32_t eb = TS * (static_cast<uint32_t>(Ly.off) + ox) + pz * zo + py * oy;
                     Calc al = Calc(0);
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                     tb_collide<L, M, BRINK>(a, f, al);
                     static_for<0, Q>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
-                        store_stream1<T>(tb_ptr(dst + q * plane, eb), IO::template from_calc<q>(f[q]));
+                        store_stream1<T>(tb_ptr(dst + q * plane, eb), IO::template store<q>(f[q]));
                     });
                 }
             }

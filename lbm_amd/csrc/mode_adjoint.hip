@@ -1,4 +1,4 @@
-// Adjoint, objective and fp64 dual-number (forward-mode) kernels, D3Q19.
+// Adjoint, objective and fp64 dual-number (forward-mode) kernels, D3Q15/19/27.
 #include "kernels_adjoint.hpp"
 #include "ops_adjoint.hpp"
 #include "ops_impl.hpp"
@@ -59,15 +59,18 @@ struct AdjImpl {
     }
 };
 
-const AdjOps kF32 = AdjImpl<D3Q19, Mode::F32>::make();
-const AdjOps kF64S = AdjImpl<D3Q19, Mode::F64S>::make();
+const AdjOps kF32[3] = {AdjImpl<D3Q15, Mode::F32>::make(), AdjImpl<D3Q19, Mode::F32>::make(),
+                        AdjImpl<D3Q27, Mode::F32>::make()};
+const AdjOps kF64S[3] = {AdjImpl<D3Q15, Mode::F64S>::make(), AdjImpl<D3Q19, Mode::F64S>::make(),
+                         AdjImpl<D3Q27, Mode::F64S>::make()};
 
 }  // namespace
 
 const AdjOps* adj_ops(int q, Mode m) {
-    if (q != 19) return nullptr;
-    if (m == Mode::F32) return &kF32;
-    if (m == Mode::F64S) return &kF64S;
+    const int i = q == 15 ? 0 : (q == 19 ? 1 : (q == 27 ? 2 : -1));
+    if (i < 0) return nullptr;
+    if (m == Mode::F32) return &kF32[i];
+    if (m == Mode::F64S) return &kF64S[i];
     return nullptr;
 }
 

@@ -12,6 +12,7 @@ __device__ __forceinline__ StepArgs view_args(const BatchView& v, int parity, in
     a.src = v.buf[parity];
     a.dst = v.buf[parity ^ 1];
     a.kind = v.kind;
+    a.lmask = v.lmask;
     a.conds = v.conds;
     a.materialised = mat;
     a.tau = v.tau;
@@ -38,16 +39,19 @@ __global__ void __launch_bounds__(256) k_batch_bulk(BatchArgs b) {
         const int z = 1 + r / v.nby;
         const StepArgs a = view_args(v, b.parity, b.materialised);
         const int x0 = 1 + ((bx << bxl) + tx) * V, y = 1 + by * by_n + ty;
+        bool finite;
         if constexpr (FUSED) {
             bool nl[V];
 #pragma unroll
             for (int i = 0; i < V; ++i) nl[i] = false;
-            bulk_cells<L, M, V, true, false>(a, x0, y, z, nl);
+            finite = bulk_cells<L, M, V, true, false>(a, x0, y, z, nl);
             for (int i = 0; i < V; ++i)
-                if (nl[i]) near_cell<L, M>(a, x0 + i, y, z);
+                if (nl[i]) finite &= near_cell<L, M>(a, x0 + i, y, z);
         } else {
-            bulk_cells<L, M, V, true, false>(a, x0, y, z);
+            finite = bulk_cells<L, M, V, true, false>(a, x0, y, z);
         }
+        // every lane that saw a non-finite value writes the same flag
+        if (!finite) b.status[tr.dom] = 1;
     }
 }
 
@@ -78,7 +82,7 @@ __global__ void __launch_bounds__(256) k_batch_cells(BatchArgs b) {
                 x -= v.L.xl;
             }
             const StepArgs a = view_args(v, b.parity, b.materialised);
-            any_cell<L, M, FS, B32>(a, 1 + x, 1 + y, z);
+            if (!any_cell<L, M, FS, B32>(a, 1 + x, 1 + y, z)) b.status[tr.dom] = 1;
         }
     }
 }
@@ -89,7 +93,7 @@ __global__ void __launch_bounds__(128) k_batch_near(BatchArgs b) {
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const NearRef n = b.near[i];
         const StepArgs a = view_args(b.views[n.dom], b.parity, b.materialised);
-        near_cell<L, M>(a, n.x, n.y, n.z);
+        if (!near_cell<L, M>(a, n.x, n.y, n.z)) b.status[n.dom] = 1;
     }
 }
 

@@ -57,6 +57,7 @@ __global__ void __launch_bounds__(256) k_badj_recompute(BatchAdjArgs b) {
         a.src = slot<T>(v, b.src_slot);
         a.dst = slot<T>(v, b.dst_slot);
         a.kind = v.kind;
+        a.lmask = v.lmask;
         a.conds = v.conds;
         a.materialised = b.materialised;
         a.tau = v.tau;
@@ -81,7 +82,7 @@ __global__ void __launch_bounds__(256) k_badj_backward(BatchAdjArgs b) {
         int x, y, z;
         if (!tile_cell(v, tr.local, lane, x, y, z)) continue;
         const auto e = elem_at<B32>(v.L, x, y, z);
-        const uint8_t k = v.kind[e];
+        const uint8_t k = kind_at(v.kind, e);
         if (k >= KIND_COND0) continue;
         AdjArgs a{};
         a.L = v.L;

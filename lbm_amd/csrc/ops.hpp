@@ -54,6 +54,11 @@ struct StepPlan {
     // Two steps per pass (k_bulk2): 64 x 8 cell tiles in x-y, tb_zc planes
     // per block along z.
     bool pairs;
+    // Two steps per pass for domains with boundary cells (k_emit_rules +
+    // k_wall2), same tiles and chunks; the first step after set-up stays a
+    // single step. wrules: the boundary rule families built in (RULE_*).
+    bool wpairs;
+    int wrules;
     int tb_ntx, tb_nty, tb_zc;
     int tb_nxcd;  // L2 domains the blocks are grouped for (1: identity order)
     int64_t tb_nblocks;
@@ -73,6 +78,9 @@ struct ModeOps {
     hipError_t (*step2)(const StepArgs& a, const StepPlan& p, hipStream_t s, EventLog* log);
     // blocks of the two-step pass resident per CU (0 where not built)
     int (*step2_blocks_per_cu)(bool brink);
+    // two steps in one pass for domains with boundary cells (null where not built)
+    hipError_t (*step2w)(const StepArgs& a, const StepPlan& p, hipStream_t s, EventLog* log);
+    int (*step2w_blocks_per_cu)(int rules);
     hipError_t (*get_pdf)(const FieldLayout& Ly, const void* src, double* out, int nx, int ny,
                           int nz, hipStream_t s);
     hipError_t (*set_pdf)(const FieldLayout& Ly, void* dst, const double* in, int nx, int ny,

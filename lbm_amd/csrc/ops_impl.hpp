@@ -6,6 +6,7 @@
 #include "kernels_aa.hpp"
 #include "kernels_fields.hpp"
 #include "kernels_tb.hpp"
+#include "kernels_wall2.hpp"
 #include "ops.hpp"
 
 namespace lbm_amd {
@@ -91,6 +92,43 @@ struct OpsImpl {
         } else {
             return hipErrorInvalidValue;
         }
+    }
+
+    // the walled pass: fp32 storage (boundary values cross memory and LDS in
+    // the arithmetic type) whose LDS slots fit 64 KiB
+    static constexpr bool wall2_built() { return M == Mode::F32 && tb_lds_bytes<L, M>() <= 65536; }
+    template <int RULES>
+    static void launch_wall2(const StepArgs& a, const StepPlan& p, hipStream_t s) {
+        if (a.nb > 0) {
+            // boundary links of the source buffer, from its fluid cells
+            StepArgs e = a;
+            e.dst = const_cast<void*>(a.src);
+            hipLaunchKernelGGL((k_emit_rules<L, M, RULES>), dim3(static_cast<uint32_t>((a.nb + 255) / 256)), dim3(256),
+                               0, s, e);
+        }
+        hipLaunchKernelGGL((k_wall2<L, M, RULES>), dim3(static_cast<uint32_t>(p.tb_nblocks)), dim3(256), 0, s, a,
+                           p.tb_ntx, p.tb_nty, p.tb_zc, p.tb_nxcd);
+    }
+    static hipError_t step2w_impl(const StepArgs& a, const StepPlan& p, hipStream_t s, EventLog* log) {
+        if constexpr (wall2_built()) {
+            if (p.wrules == (RULE_MOVINGWALL))
+                launch_wall2<RULE_MOVINGWALL>(a, p, s);
+            else
+                launch_wall2<RULES_ALL>(a, p, s);
+            if (log) log->mark(s, 0);
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
+    static int step2w_occupancy(int rules) {
+        int n = 0;
+        if constexpr (wall2_built()) {
+            const void* k = rules == RULE_MOVINGWALL ? reinterpret_cast<const void*>(&k_wall2<L, M, RULE_MOVINGWALL>)
+                                                     : reinterpret_cast<const void*>(&k_wall2<L, M, RULES_ALL>);
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, 256, 0) != hipSuccess) n = 0;
+        }
+        return n;
     }
 
     static int step2_occupancy(bool brink) {
@@ -190,6 +228,8 @@ struct OpsImpl {
         o.step = &step_impl<V0, Vs...>;
         o.step2 = tb_built() ? &step2_impl : nullptr;
         o.step2_blocks_per_cu = &step2_occupancy;
+        o.step2w = wall2_built() ? &step2w_impl : nullptr;
+        o.step2w_blocks_per_cu = &step2w_occupancy;
         o.get_pdf = &get_pdf;
         o.set_pdf = &set_pdf;
         o.moments = &moments;

@@ -4,7 +4,8 @@
 // (the parity mode). The other modes keep the shifted value g_q = f_q - w_q,
 // multiplied by a per-direction power of two for the 16-bit formats, and do
 // their arithmetic in the Calc type (float, or double for F64S).
-// Conversions round to nearest even (v_cvt_f16_f32, v_cvt_pk_bf16_f32).
+// Conversions round to nearest even (v_cvt_f16_f32, v_cvt_pk_bf16_f32). The
+// 16-bit formats are the compiler's arithmetic types _Float16 and __bf16.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,7 +50,7 @@ struct StorageOf<Mode::F32> {
 };
 template <>
 struct StorageOf<Mode::F16> {
-    using T = __half;
+    using T = _Float16;
     using Calc = float;
     static constexpr bool shifted = true;
 };
@@ -64,57 +65,75 @@ struct StorageOf<Mode::BF16> {
 template <Mode M>
 constexpr bool kRcp = sizeof(typename StorageOf<M>::T) == 2;
 
-// Per-direction scale of the 16-bit formats: 2^8 times the power of two
-// nearest to 1/w_q. |g_q / w_q| stays below ~1 for stable flows, so stored
-// magnitudes sit near 2^8, far from the fp16 subnormal range (< 2^-14) and
-// with 2^8 of headroom below the fp16 maximum (65504). Powers of two keep the
-// scaling itself exact.
-template <class L>
-LBM_HD constexpr int scale_exp(int q) {
-    // smallest e with 2^e >= (wden / wnum) / sqrt(2), compared in integers
-    int e = 0;
-    const int num = L::wnum(q), den = L::wden;
-    while (2 * (num << e) * (num << e) < den * den) ++e;
-    return e + 8;
-}
+// Scale of the stored values: fp16 stores g * 2^12, every other format g
+// itself. fp16's normal range is [2^-14, 65504], so stored magnitudes of
+// 2^-14 .. 2^16 cover |g| from 1.5e-8 to 16: stable flows keep |g_q| below
+// w_q <= 1/3 (|g_q / w_q| < 1), far from the top, and values below the
+// bottom are under the format's own rounding of the usual |g| ~ 1e-4 .. 1e-2.
+// bf16 has fp32's exponent range and needs no scale. A power of two keeps the
+// scaling exact, and one scale for all directions lets the collision work on
+// the stored (scaled) values directly: scaled sums equal the scaled sums of
+// the unscaled values, bit for bit.
+template <Mode M>
+constexpr int kStoreScaleExp = (M == Mode::F16) ? 12 : 0;
 
-template <class L>
-LBM_HD constexpr float scale_of(int q) {
-    return static_cast<float>(1u << scale_exp<L>(q));
-}
-
-template <class L>
-LBM_HD constexpr float inv_scale_of(int q) {
-    return 1.0f / static_cast<float>(1u << scale_exp<L>(q));
-}
-
-// Load / store of one stored value as Calc, with the scale applied.
+// Conversions of one stored value.
+//   to_calc / from_calc: the shifted value g itself (boundary rules,
+//     moments, field I/O);
+//   load / store: the collision's representation, g * 2^SX (the stored
+//     scale), which the step kernels pass through collide_shifted<..., SX>;
+//   scaled: g -> g * 2^SX, for rule values entering a collision.
 template <Mode M, class L>
 struct Io {
     using S = StorageOf<M>;
     using T = typename S::T;
     using Calc = typename S::Calc;
-    static constexpr bool scaled = (M == Mode::F16 || M == Mode::BF16);
+    static constexpr int SX = kStoreScaleExp<M>;
+    static constexpr float kScale = static_cast<float>(1u << SX);
+    static constexpr float kInvScale = 1.0f / static_cast<float>(1u << SX);
 
     template <int q>
-    __device__ __forceinline__ static Calc to_calc(T v) {
+    __device__ __forceinline__ static Calc load(T v) {
         if constexpr (M == Mode::F16) {
-            return __half2float(v) * inv_scale_of<L>(q);
+            return static_cast<float>(v);
         } else if constexpr (M == Mode::BF16) {
-            return static_cast<float>(v) * inv_scale_of<L>(q);
+            return static_cast<float>(v);
         } else {
             return static_cast<Calc>(v);
         }
     }
     template <int q>
-    __device__ __forceinline__ static T from_calc(Calc c) {
+    __device__ __forceinline__ static T store(Calc c) {
         if constexpr (M == Mode::F16) {
-            return __float2half_rn(c * scale_of<L>(q));
+            // c is rounded to fp32 first: without this (empty) barrier the
+            // compiler may fuse an fma producing c with the conversion into
+            // one mixed-precision fma that rounds straight to fp16, in some
+            // kernels and not in others, and their results would differ
+            __asm__("" : "+v"(c));
+            return static_cast<_Float16>(c);  // round to nearest even
         } else if constexpr (M == Mode::BF16) {
-            return static_cast<__bf16>(c * scale_of<L>(q));
+            return static_cast<__bf16>(c);
         } else {
             return static_cast<T>(c);
         }
+    }
+    template <int q>
+    __device__ __forceinline__ static Calc scaled(Calc g) {
+        if constexpr (SX != 0)
+            return g * kScale;
+        else
+            return g;
+    }
+    template <int q>
+    __device__ __forceinline__ static Calc to_calc(T v) {
+        if constexpr (SX != 0)
+            return load<q>(v) * kInvScale;
+        else
+            return load<q>(v);
+    }
+    template <int q>
+    __device__ __forceinline__ static T from_calc(Calc c) {
+        return store<q>(scaled<q>(c));
     }
 };
 

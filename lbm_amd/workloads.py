@@ -39,16 +39,21 @@ def sphere_mask(xl, yl, zl, solid_fraction=0.2, rmin=6.0, rmax=12.0, seed=12345,
     solid_fraction. `margin` keeps centres away from the inflow/outflow faces."""
     rng = np.random.default_rng(seed)
     solid = np.zeros((zl, yl, xl), dtype=bool)
-    z, y, x = np.meshgrid(np.arange(1, zl + 1), np.arange(1, yl + 1), np.arange(1, xl + 1), indexing="ij")
+    ntot = solid.size
+    nsolid = 0  # kept equal to solid.sum() as spheres are added
     m = rmax if margin is None else margin
-    while solid.mean() < solid_fraction:
+    while nsolid / ntot < solid_fraction:
         cx = rng.uniform(1 + m, xl - m)
         cy, cz = rng.uniform(1, yl), rng.uniform(1, zl)
         r = rng.uniform(rmin, rmax)
         lo = [max(1, int(np.floor(c - r))) for c in (cz, cy, cx)]
         hi = [min(n, int(np.ceil(c + r))) for c, n in ((cz, zl), (cy, yl), (cx, xl))]
         sl = tuple(slice(a - 1, b) for a, b in zip(lo, hi))
-        solid[sl] |= (x[sl] - cx) ** 2 + (y[sl] - cy) ** 2 + (z[sl] - cz) ** 2 <= r * r
+        z, y, x = np.meshgrid(np.arange(lo[0], hi[0] + 1), np.arange(lo[1], hi[1] + 1),
+                              np.arange(lo[2], hi[2] + 1), indexing="ij")
+        before = int(solid[sl].sum())
+        solid[sl] |= (x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 <= r * r
+        nsolid += int(solid[sl].sum()) - before
     return (~solid).astype(np.uint8)
 
 

@@ -38,7 +38,7 @@ def test_binding_covers_every_declared_symbol():
 
 def test_host_only_calls():
     lib = capi.load()
-    assert lib.lbm_abi_version() == 4
+    assert lib.lbm_abi_version() == 5
     d = capi.Desc()
     assert lib.lbm_desc_init(ctypes.byref(d)) == 0
     assert d.q == 19 and d.tau == 1.0 and d.spacing[2] == 1.0
@@ -114,3 +114,12 @@ def test_config_file(tmp_path):
     p.write_text("tau = 0.6\n")
     with pytest.raises(ValueError, match="required"):
         scenario.parse_config_file(p)
+
+
+def test_binding_checks_array_sizes():
+    """Host arrays handed to the C ABI are checked for their size before the
+    call (the library reads exactly that many values from the pointer)."""
+    v = capi._f64(np.arange(6).reshape(2, 3), 6, "x")
+    assert v.dtype == np.float64 and v.shape == (6,) and v.flags.c_contiguous
+    with pytest.raises(ValueError, match="porosity direction: expected 8 values, got 7"):
+        capi._f64(np.zeros(7), 8, "porosity direction")

@@ -142,6 +142,44 @@ def test_driver_vtk_mask_scenario(tmp_path):
     assert np.array_equal(rho[fluid], ro[fluid]) and np.array_equal(u[fluid], uo[fluid])
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("layout", ["ab", "aa"])
+def test_driver_reference_pipe_block(tmp_path, layout):
+    """The reference's pipe scenario (build/scenarios/pipe.xml:4-11: noslip
+    z0/zmax/y0/ymax, inflow vx = 0.03 at x0, outflow at xmax) on the committed
+    block of its own mask build/scenarios/pipe.vtk (x = 80 .. 119 around the
+    constriction, tests/golden/pipe_crop.vtk.gz): the driver's .vts output
+    equals the fp64 oracle bit for bit on every fluid cell."""
+    import gzip
+    print(f"driver reference pipe block, layout {layout}", flush=True)
+    vtk = tmp_path / "pipe_crop.vtk"
+    vtk.write_bytes(gzip.decompress((ROOT / "tests" / "golden" / "pipe_crop.vtk.gz").read_bytes()))
+    scen = tmp_path / "pipe.xml"
+    scen.write_text(f"""<?xml version="1.0" ?>
+<scenario name="Pipe flow">
+   <domain vtk-file="{vtk}">
+        <boundary extent="z0" condition="noslip" />
+        <boundary extent="zmax"    condition="noslip" />
+        <boundary extent="x0"   condition="inflow" vx="0.03" vy="0" vz="0" />
+        <boundary extent="xmax"  condition="outflow" />
+        <boundary extent="y0"  condition="noslip" />
+        <boundary extent="ymax"   condition="noslip" />
+    </domain>
+</scenario>
+""")
+    steps = 60
+    cfg = write_cfg(tmp_path, scen, steps, steps, f"layout = {layout}\n")
+    r = run([cfg], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    sc = scenario.parse_scenario_file(scen)
+    assert (sc.xl, sc.yl, sc.zl) == (40, 54, 54) and int(sc.fluid_mask.sum()) == 29649
+    rho, u = read_vts(tmp_path / "vtk" / f"Pipe flow.{steps}.vts")
+    ro, uo, fluid = oracle_moments(sc, steps)
+    assert fluid.sum() == 29649
+    assert np.abs(uo[fluid, 0]).max() > 0.01  # inflow reached through the constriction's inlet side
+    assert np.array_equal(rho[fluid], ro[fluid]) and np.array_equal(u[fluid], uo[fluid])
+
+
 REF_MAIN = Path("/root/reference/src/main.cpp")
 
 

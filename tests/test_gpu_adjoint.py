@@ -191,3 +191,76 @@ def test_c4_inverse_problem_gradient():
     # against central differences (which carry an O(eps^2) truncation error of
     # their own), relative to the size of the directional derivatives
     assert fd_err <= 1e-4
+
+
+def channel(q, storage, alpha, outlet, shape=(14, 7, 6), tau=0.7, u_in=0.03):
+    """Channel along x with a porosity field: inflow at x0, outflow (or a
+    pressure face) at xmax, a no-slip wall at y0, a moving wall at ymax
+    (velocity along x and z) and periodic z."""
+    d = capi.Domain(*shape, tau, q=q, storage=storage, periodic=(0, 0, 1))
+    sc = scenario.Scenario("channel", *shape)
+    faces = [("y0", capi.NOSLIP, (0.0, 0.0, 0.0, 1.0)), ("ymax", capi.MOVINGWALL, (0.02, 0.0, 0.01, 1.0)),
+             ("x0", capi.INFLOW, (u_in, 0.0, 0.0, 1.0)),
+             ("xmax", outlet, (0.0, 0.0, 0.0, 1.0 if outlet == capi.OUTFLOW else 0.997))]
+    for ext, kind, p in faces:
+        sc.boundaries.append(scenario.Boundary(kind, p, scenario.face_box(ext, *shape), "", ext))
+    sc.apply(d)
+    d.set_brinkman(alpha, (0.0, 0.0, 0.0))
+    d.finalize()
+    return d
+
+
+@pytest.mark.parametrize("outlet", ["outflow", "pressure"])
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_adjoint_outflow_pressure_against_jvp_and_differences(q, outlet):
+    """Adjoint rules of the Outflow and Pressure boundaries (velocity at the
+    reference density, equilibrium pair minus the bounced value), with
+    Inflow, NoSlip and MovingWall on the same channel, for every lattice:
+    porosity directional derivatives agree with the fp64 JVP to 1e-10, and
+    with central differences of the objective per direction to 1e-6; the
+    moving-wall velocity gradient agrees with central differences."""
+    kind = capi.OUTFLOW if outlet == "outflow" else capi.PRESSURE
+    shape, steps = (14, 7, 6), 30
+    rng = np.random.default_rng(100 + q)
+    alpha = rng.uniform(0.0, 0.2, shape[::-1])
+    target = 1e-3 * rng.standard_normal((shape[1], shape[2], 3))
+    say(f"adjoint {outlet} channel q={q}")
+    d = channel(q, "f64s", alpha, kind)
+    tape = capi.Tape(d, steps, capi.OBJ_PLANE_VELOCITY_MISFIT, plane_x=shape[0] - 1, target_u=target,
+                     checkpoint_every=5)
+    J = tape.forward(alpha)
+    grad, gwall = tape.backward(1.0)
+    eps = 1e-5
+    for k in range(3):
+        da = rng.standard_normal(alpha.shape)
+        Jd, dJ = tape.jvp(alpha, da)
+        adj = float((grad * da).sum())
+        jp = tape.forward(alpha + eps * da)
+        jm = tape.forward(alpha - eps * da)
+        fd = (jp - jm) / (2 * eps)
+        say(f"  dir {k}: adjoint {adj:.12e} jvp {dJ:.12e} fd {fd:.12e}")
+        assert rel(Jd, J) < 1e-13
+        assert rel(adj, dJ) < 1e-10
+        assert rel(adj, fd) < 1e-6
+    # moving-wall velocity, x component, by central differences
+    js = []
+    for s in (1, -1):
+        dd = capi.Domain(*shape, 0.7, q=q, storage="f64s", periodic=(0, 0, 1))
+        sc = scenario.Scenario("channel", *shape)
+        for ext, kk, p in [("y0", capi.NOSLIP, (0.0, 0.0, 0.0, 1.0)),
+                           ("ymax", capi.MOVINGWALL, (0.02 + s * eps, 0.0, 0.01, 1.0)),
+                           ("x0", capi.INFLOW, (0.03, 0.0, 0.0, 1.0)),
+                           ("xmax", kind, (0.0, 0.0, 0.0, 1.0 if kind == capi.OUTFLOW else 0.997))]:
+            sc.boundaries.append(scenario.Boundary(kk, p, scenario.face_box(ext, *shape), "", ext))
+        sc.apply(dd)
+        dd.set_brinkman(alpha, (0.0, 0.0, 0.0))
+        dd.finalize()
+        t2 = capi.Tape(dd, steps, capi.OBJ_PLANE_VELOCITY_MISFIT, plane_x=shape[0] - 1, target_u=target)
+        js.append(t2.forward(alpha))
+        t2.close()
+        dd.close()
+    fdw = (js[0] - js[1]) / (2 * eps)
+    say(f"  dJ/du_wall,x adjoint {gwall[0]:.12e} fd {fdw:.12e}")
+    assert rel(gwall[0], fdw) < 1e-6
+    tape.close()
+    d.close()

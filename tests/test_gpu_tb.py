@@ -77,7 +77,7 @@ def test_two_step_pass_equals_single_steps(storage, q):
                                     f"{int((~same).sum())} values differ")
 
 
-def test_pass_applies_only_to_bulk_only_fp32_fp16_bf16_domains():
+def test_pass_applies_to_fp32_fp16_bf16_two_buffer_domains():
     say("two-step pass applicability")
     per = capi.Domain(70, 9, 6, 0.8, q=19, storage="f32", periodic=(1, 1, 1))
     per.finalize()
@@ -88,10 +88,11 @@ def test_pass_applies_only_to_bulk_only_fp32_fp16_bf16_domains():
     assert per.info()["steps_per_pass"] == 2 and per.info()["z_chunk"] == 5
     with pytest.raises(capi.LbmError):
         per.set_time_blocking(3)
+    # walled domains take the walled pass (tests/test_gpu_wall2.py)
     walled = capi.Domain(70, 9, 6, 0.8, q=19, storage="f32", periodic=(1, 1, 0))
     scenario.cavity(70, 9, 6).apply(walled)
     walled.finalize()
-    assert walled.info()["steps_per_pass"] == 1
+    assert walled.info()["steps_per_pass"] == 2
     short = capi.Domain(32, 9, 6, 0.8, q=19, storage="f16", periodic=(1, 1, 1))
     short.finalize()
     assert short.info()["steps_per_pass"] == 1  # 16-bit rows of 32 cells: single steps
