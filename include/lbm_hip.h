@@ -154,6 +154,14 @@ int lbm_init_equilibrium(lbm_domain* d, const double* rho, const double* u);
  * as the VTK writer evaluates them (include/io/vtk.hpp:62-73): rho has
  * xl*yl*zl doubles, u 3*xl*yl*zl, x fastest. Either may be NULL. */
 int lbm_moments(lbm_domain* d, double* rho, double* u);
+/* lbm_moments for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same outputs, bit for bit, with the device staging of the moments in a
+ * fixed per-GPU array of 2^20 cells (32 MiB) instead of a per-call
+ * allocation. Domains with more interior cells are rejected
+ * (LBM_ERR_UNSUPPORTED). Calls on one GPU share that array, so calls that
+ * overlap on one GPU race. No counterpart in the reference. */
+int lbm_moments_noalloc(lbm_domain* d, double* rho, double* u);
 /* Extension: fp64 sums over interior fluid cells of rho and of the momentum
  * (x, y, z), reduced on the device. */
 int lbm_conservation(lbm_domain* d, double out[4]);

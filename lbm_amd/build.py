@@ -40,6 +40,7 @@ COMMON = [
 NOFMA = ["-ffp-contract=off"]
 UNITS = {
     "capi.hip": [],
+    "capi_noalloc.hip": [],
     "capi_adjoint.hip": [],
     "mode_adjoint.hip": NOFMA,
     "capi_batch.hip": [],

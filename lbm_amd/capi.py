@@ -67,6 +67,7 @@ SIGNATURES = [
     ("lbm_set_pdf", c_int, [c_void_p, POINTER(c_double)]),
     ("lbm_init_equilibrium", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
     ("lbm_moments", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
+    ("lbm_moments_noalloc", c_int, [c_void_p, POINTER(c_double), POINTER(c_double)]),
     ("lbm_conservation", c_int, [c_void_p, POINTER(c_double)]),
     ("lbm_time_steps", c_int, [c_void_p, c_uint64, POINTER(c_double)]),
     ("lbm_copy_bandwidth", c_int, [c_int, c_uint64, c_int, POINTER(c_double)]),
@@ -261,6 +262,14 @@ class Domain:
         rho = np.empty((self.zl, self.yl, self.xl), dtype=np.float64)
         u = np.empty((self.zl, self.yl, self.xl, 3), dtype=np.float64)
         _check(load().lbm_moments(self._h, _dptr(rho), _dptr(u)))
+        return rho, u
+
+    def moments_noalloc(self):
+        """moments() through lbm_moments_noalloc (fixed per-GPU staging, no
+        device allocation; at most 2^20 interior cells)."""
+        rho = np.empty((self.zl, self.yl, self.xl), dtype=np.float64)
+        u = np.empty((self.zl, self.yl, self.xl, 3), dtype=np.float64)
+        _check(load().lbm_moments_noalloc(self._h, _dptr(rho), _dptr(u)))
         return rho, u
 
     def conservation(self) -> np.ndarray:
