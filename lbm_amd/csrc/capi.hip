@@ -203,6 +203,7 @@ void free_device(lbm_domain* d) {
     d->buf[0] = d->buf[1] = nullptr;
     d->d_kind = nullptr;
     d->d_lmask = nullptr;
+    d->d_lcond = nullptr;
     d->d_blist = nullptr;
     d->d_conds = nullptr;
     d->d_alpha = nullptr;
@@ -484,10 +485,12 @@ int lbm_finalize(lbm_domain* d) {
         }
     }
 
-    // Cell kinds in the padded device layout, the NEAR list, and the link
-    // masks of NEAR cells (bit q: the source x - c_q is not fluid).
+    // Cell kinds in the padded device layout, the NEAR list, the link masks
+    // of NEAR cells (bit q: the source x - c_q is not fluid) and the
+    // condition their links share (LCOND_MIXED where they differ).
     std::vector<uint8_t> kind(static_cast<size_t>(L.plane), KIND_INERT);
     std::vector<uint32_t> lmask(static_cast<size_t>(L.plane), 0u);
+    std::vector<uint8_t> lcond(static_cast<size_t>(L.plane), LCOND_MIXED);
     std::vector<CellCoord> blist;
     uint64_t nfluid = 0, nbulk = 0, nonbulk_interior = 0;
     for (int z = 0; z < d->nz; ++z)
@@ -504,15 +507,22 @@ int lbm_finalize(lbm_domain* d) {
                 if (!in) continue;  // unassigned ghost: INERT
                 ++nfluid;
                 uint32_t m = 0;
+                int shared = -1;  // condition of the links so far (-2: more than one)
                 for (int q = 0; q < Q; ++q) {
                     const int64_t sx = d->wrapc(x - lat.c[q][0], 0);
                     const int64_t sy = d->wrapc(y - lat.c[q][1], 1);
                     const int64_t sz = d->wrapc(z - lat.c[q][2], 2);
-                    if (d->handler[d->hidx(sx, sy, sz)] != 0) m |= 1u << q;
+                    const uint16_t hs = d->handler[d->hidx(sx, sy, sz)];
+                    if (hs != 0) {
+                        m |= 1u << q;
+                        const int ci = cond_map[hs - 1];
+                        shared = shared == -1 ? ci : (shared == ci ? ci : -2);
+                    }
                 }
                 if (m) {
                     kind[e] = KIND_NEAR;
                     lmask[e] = m;
+                    if (shared >= 0) lcond[e] = static_cast<uint8_t>(shared);
                     blist.push_back(CellCoord{x, y, z, 0});
                     ++nonbulk_interior;
                 } else {
@@ -523,11 +533,14 @@ int lbm_finalize(lbm_domain* d) {
 
     HIPCHK(hipMemcpyAsync(d->d_kind, kind.data(), kind.size(), hipMemcpyHostToDevice, d->stream));
     if (!d->d_lmask) {
-        HIPCHK(hipMalloc(&d->d_lmask, lmask.size() * sizeof(uint32_t)));
-        d->device_bytes += lmask.size() * sizeof(uint32_t);
+        const size_t bytes = lmask.size() * (sizeof(uint32_t) + sizeof(uint8_t));
+        HIPCHK(hipMalloc(&d->d_lmask, bytes));
+        d->d_lcond = reinterpret_cast<uint8_t*>(d->d_lmask + lmask.size());
+        d->device_bytes += bytes;
     }
     HIPCHK(hipMemcpyAsync(d->d_lmask, lmask.data(), lmask.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                           d->stream));
+    HIPCHK(hipMemcpyAsync(d->d_lcond, lcond.data(), lcond.size(), hipMemcpyHostToDevice, d->stream));
     d->nb = static_cast<int64_t>(blist.size());
     if (d->nb) {
         HIPCHK(hipMalloc(&d->d_blist, blist.size() * sizeof(CellCoord)));

@@ -136,6 +136,7 @@ struct lbm_domain {
     int cur = 0;
     uint8_t* d_kind = nullptr;
     uint32_t* d_lmask = nullptr;  // per element: non-fluid sources of a NEAR cell (bit q)
+    uint8_t* d_lcond = nullptr;   // per element: their shared condition (same allocation, after d_lmask)
     CellCoord* d_blist = nullptr;
     int64_t nb = 0;
     Cond* d_conds = nullptr;
@@ -177,6 +178,7 @@ struct lbm_domain {
         a.dst = dst;
         a.kind = d_kind;
         a.lmask = d_lmask;
+        a.lcond = d_lcond;
         a.blist = d_blist;
         a.nb = nb;
         a.conds = d_conds;

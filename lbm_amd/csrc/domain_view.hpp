@@ -26,6 +26,7 @@ enum : uint8_t {
     KIND_INERT = 254,   // never read (unassigned ghost, unreachable solid)
 };
 constexpr int kMaxConds = KIND_INERT - KIND_COND0;  // 252 distinct conditions
+constexpr uint8_t LCOND_MIXED = 0xFF;  // links of a NEAR cell under more than one condition
 
 // Boundary-condition kinds, numbered as in include/lbm_hip.h.
 enum : int {
@@ -78,6 +79,9 @@ struct StepArgs {
     // per element: bit q set when the cell is fluid and its source x - c_q is
     // not (the links a near-boundary cell takes from boundary rules)
     const uint32_t* lmask;
+    // per element: for a NEAR cell, the condition index shared by every link
+    // of its mask, or LCOND_MIXED where they differ (edges and corners)
+    const uint8_t* lcond;
     const CellCoord* blist;  // cells of kind NEAR
     int64_t nb;
     const Cond* conds;

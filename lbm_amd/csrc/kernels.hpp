@@ -366,6 +366,35 @@ __device__ __forceinline__ OwnSums<typename Io<M, L>::Calc> own_sums_at(
     return acc.r;
 }
 
+// The MovingWall and Outflow/Pressure values of the fp32/fp64 storage modes,
+// shared by rule_value and rule_value_flat so that both round alike.
+// MovingWall (include/boundary.hpp:55-65) with c_q . u_wall tabulated per
+// condition.
+template <class L, class Calc, int q>
+__device__ __forceinline__ Calc moving_wall_rule(const OwnSums<Calc>& os, Calc finv, const GCond& cd) {
+    const Calc rho = Calc(1) + os.s;
+    Calc cu;
+    if constexpr (std::is_same<Calc, float>::value)
+        cu = cd.cu32[q];
+    else
+        cu = static_cast<Calc>(cd.cu64[q]);
+    return finv + static_cast<Calc>(6.0 * weight<L>(q)) * rho * cu;
+}
+
+// Outflow / Pressure (include/boundary.hpp:140-147, 206-211):
+// (feq_q - w) + (feq_qbar - w) - finv at the condition's density and x's
+// velocity; the odd part of the two equilibria cancels.
+template <class L, class Calc, int q>
+__device__ __forceinline__ Calc outflow_rule(const OwnSums<Calc>& os, Calc finv, const GCond& cd) {
+    const Calc rr = static_cast<Calc>(cd.rho);
+    const Calc ir = Calc(1) / rr;
+    const Calc ux = os.m0 * ir, uy = os.m1 * ir, uz = os.m2 * ir;
+    const Calc cu = Calc(L::cx(q)) * ux + Calc(L::cy(q)) * uy + Calc(L::cz(q)) * uz;
+    const Calc uu = ux * ux + uy * uy + uz * uz;
+    const Calc even = (rr - Calc(1)) + rr * (Calc(4.5) * cu * cu - Calc(1.5) * uu);
+    return static_cast<Calc>(2.0 * weight<L>(q)) * even - finv;
+}
+
 // Value of a NoSlip, MovingWall, Inflow, Outflow or Pressure rule for
 // direction q, from x's own sums and own[qbar] (finv).
 template <class L, Mode M, int q, int RULES = RULES_ALL>
@@ -385,13 +414,7 @@ __device__ __forceinline__ typename Io<M, L>::Calc rule_value(const GCond& cd,
                 constexpr double cs = kCS;
                 return finv + 2.0 * weight<L>(q) * os.s * c_dot_u / (cs * cs);
             } else {
-                const Calc rho = Calc(1) + os.s;
-                Calc cu;
-                if constexpr (std::is_same<Calc, float>::value)
-                    cu = cd.cu32[q];
-                else
-                    cu = static_cast<Calc>(cd.cu64[q]);
-                return finv + static_cast<Calc>(6.0 * weight<L>(q)) * rho * cu;
+                return moving_wall_rule<L, Calc, q>(os, finv, cd);
             }
         }
     }
@@ -407,18 +430,38 @@ __device__ __forceinline__ typename Io<M, L>::Calc rule_value(const GCond& cd,
                 const double fb = feq_exact<L, Q - 1 - q>(cd.rho, u);
                 return fq + fb - finv;
             } else {
-                const Calc rr = static_cast<Calc>(cd.rho);
-                const Calc ir = Calc(1) / rr;
-                const Calc ux = os.m0 * ir, uy = os.m1 * ir, uz = os.m2 * ir;
-                const Calc cu = Calc(L::cx(q)) * ux + Calc(L::cy(q)) * uy + Calc(L::cz(q)) * uz;
-                const Calc uu = ux * ux + uy * uy + uz * uz;
-                // (feq_q - w) + (feq_qbar - w): the odd part cancels
-                const Calc even = (rr - Calc(1)) + rr * (Calc(4.5) * cu * cu - Calc(1.5) * uu);
-                return static_cast<Calc>(2.0 * weight<L>(q)) * even - finv;
+                return outflow_rule<L, Calc, q>(os, finv, cd);
             }
         }
     }
     return finv;  // BC_NOSLIP, boundary.hpp:26-28
+}
+
+// rule_value without branches (fp32/fp64 storage modes): every family built
+// in is evaluated and the condition's kind selects one. cd must be a valid
+// condition even where the result is not used, so a caller can issue the
+// loads of all its links at once instead of one link after another.
+template <class L, Mode M, int q, int RULES = RULES_ALL>
+__device__ __forceinline__ typename Io<M, L>::Calc rule_value_flat(const GCond& cd,
+                                                                  const OwnSums<typename Io<M, L>::Calc>& os,
+                                                                  typename Io<M, L>::Calc finv) {
+    static_assert(M != Mode::EXACT64, "the reference-order mode keeps rule_value");
+    using Calc = typename Io<M, L>::Calc;
+    const int kind = cd.kind;
+    Calc v = finv;  // BC_NOSLIP
+    if constexpr ((RULES & RULE_MOVINGWALL) != 0) {
+        const Calc w = moving_wall_rule<L, Calc, q>(os, finv, cd);
+        v = kind == BC_MOVINGWALL ? w : v;
+    }
+    if constexpr ((RULES & RULE_INFLOW) != 0) {
+        const Calc g = static_cast<Calc>(cd.geq[q]);
+        v = kind == BC_INFLOW ? g : v;
+    }
+    if constexpr ((RULES & RULE_OUTFLOW) != 0) {
+        const Calc o = outflow_rule<L, Calc, q>(os, finv, cd);
+        v = (kind == BC_OUTFLOW || kind == BC_PRESSURE) ? o : v;
+    }
+    return v;
 }
 
 // FreeSlip candidates (include/boundary.hpp:93-110), in order: which of the

@@ -69,14 +69,23 @@ __global__ void __launch_bounds__(256) k_emit_rules(StepArgs a) {
     });
 }
 
+// Waves per SIMD the pass is compiled for: as many blocks as LDS holds
+// (160 KiB per CU, one wave of each block per SIMD), at most 4 (128 VGPRs).
+template <class L, Mode M>
+constexpr int wall2_waves() {
+    const size_t by_lds = (size_t{160} << 10) / tb_lds_bytes<L, M>();
+    return by_lds < 1 ? 1 : (by_lds > 4 ? 4 : static_cast<int>(by_lds));
+}
+
 template <class L, Mode M, int RULES>
-__global__ void __launch_bounds__(256) k_wall2(StepArgs a, int ntx, int nty, int zc, int nxcd) {
+__global__ void __launch_bounds__(256, (wall2_waves<L, M>())) k_wall2(StepArgs a, int ntx, int nty, int zc, int nxcd) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
     using S = TbSlots<L>;
     constexpr int Q = L::Q;
     static_assert(sizeof(T) == sizeof(Calc), "boundary values cross memory in the arithmetic type");
+    static_assert(TB_PASS == TB_OUT + 1 && TB_NP - TB_CELLS <= 256, "tile cells on k < TB_OUT, the ring on the last k");
     __shared__ T lds[S::NSLOT][TB_CELLS];
 
     const FieldLayout& Ly = a.L;
@@ -95,58 +104,50 @@ __global__ void __launch_bounds__(256) k_wall2(StepArgs a, int ntx, int nty, int
     const uint32_t py = static_cast<uint32_t>(Ly.px) * TS, pz = static_cast<uint32_t>(Ly.pxy) * TS;
     const uint32_t off = static_cast<uint32_t>(Ly.off) * TS;
 
-    // this thread's patch cells: tile-local (pi, pj) in [-1, TX] x [-1, TY];
-    // gx, gy are their coordinates in the ghosted box
+    // This thread's patch cells, tile-local (pi, pj) in [-1, TX] x [-1, TY]:
+    // k < TB_OUT are the tile cells c = k * 256 + tid, the cells whose step
+    // 2 this thread also does; the last k is ring cell tid (the rows pj = -1
+    // and pj = TY, then the columns pi = -1 and pi = TX). gx, gy are the
+    // coordinates in the ghosted box.
     int pi[TB_PASS], pj[TB_PASS], gx[TB_PASS], gy[TB_PASS];
     bool live[TB_PASS];
 #pragma unroll
     for (int k = 0; k < TB_PASS; ++k) {
-        const int p = k * 256 + tid;
-        pi[k] = p % TB_PX - 1;
-        pj[k] = p / TB_PX - 1;
+        bool in = true;
+        if (k < TB_OUT) {
+            const int c = k * 256 + tid;
+            pi[k] = c % TB_TX;
+            pj[k] = c / TB_TX;
+        } else if (tid < TB_PX) {
+            pi[k] = tid - 1;
+            pj[k] = -1;
+        } else if (tid < 2 * TB_PX) {
+            pi[k] = tid - TB_PX - 1;
+            pj[k] = TB_TY;
+        } else if (tid < 2 * TB_PX + TB_TY) {
+            pi[k] = -1;
+            pj[k] = tid - 2 * TB_PX;
+        } else {
+            pi[k] = TB_TX;
+            pj[k] = tid - 2 * TB_PX - TB_TY;
+            in = tid < TB_NP - TB_CELLS;
+        }
         const int ux = x0 + pi[k], uy = y0 + pj[k];
-        live[k] = p < TB_NP && ux <= Ly.xl + 1 && uy <= Ly.yl + 1;
+        live[k] = in && ux <= Ly.xl + 1 && uy <= Ly.yl + 1;
         gx[k] = wrap_axis(ux, Ly.xl, Ly.per[0]);
         gy[k] = wrap_axis(uy, Ly.yl, Ly.per[1]);
     }
 
-    T hold[TB_PASS][Q];  // step-1 values of this plane's patch cells
+    T hold[TB_PASS][Q];  // step-1 values (c_z >= 0) of this plane's patch cells
     T prev[TB_PASS][Q];  // the c_z = +1 values of the previous plane's
-    bool fl[TB_PASS], pfl[TB_PASS];  // patch cell fluid on this / the previous plane
-    uint32_t wm[TB_PASS];            // link mask of a near-boundary tile cell (bit k: x - c_k not fluid)
-    // Cell kinds are loaded one plane ahead (patch cells) or at the top of
-    // the iteration (step-2 cells), so the field loads do not wait on them.
-    uint8_t kcur[TB_PASS], knext[TB_PASS];
-    const uint32_t rowb[TB_PASS] = {py * static_cast<uint32_t>(gy[0]) + TS * static_cast<uint32_t>(gx[0]),
-                                    py * static_cast<uint32_t>(gy[1]) + TS * static_cast<uint32_t>(gx[1]),
-                                    py * static_cast<uint32_t>(gy[2]) + TS * static_cast<uint32_t>(gx[2])};
-    static_assert(TB_PASS == 3, "rowb lists one entry per patch cell");
-    auto plane_off = [&](int zz) -> uint32_t {
-        return off + pz * static_cast<uint32_t>(wrap_axis(zz, Ly.zl, Ly.per[2]));
-    };
+    bool fl[TB_PASS], pfl[TB_PASS];     // patch cell fluid on this / the previous plane
+    uint32_t wm[TB_OUT];  // link masks of this plane's tile cells next to non-fluid cells
 #pragma unroll
     for (int k = 0; k < TB_PASS; ++k) {
         pfl[k] = false;
         static_for<0, Q>([&](auto qc) { prev[k][decltype(qc)::value] = T{}; });
-        knext[k] = live[k] ? kind_at(a.kind, (plane_off(z0 - 1) + rowb[k]) / TS) : uint8_t(KIND_INERT);
     }
     for (int zs = z0 - 1; zs <= z1 + 1; ++zs) {
-#pragma unroll
-        for (int k = 0; k < TB_PASS; ++k) {
-            kcur[k] = knext[k];
-            knext[k] = (live[k] && zs + 1 <= z1 + 1) ? kind_at(a.kind, (plane_off(zs + 1) + rowb[k]) / TS)
-                                                      : uint8_t(KIND_INERT);
-        }
-        uint8_t kout[TB_OUT];
-#pragma unroll
-        for (int k = 0; k < TB_OUT; ++k) {
-            const int c = k * 256 + tid;
-            const int ox = x0 + c % TB_TX, oy = y0 + c / TB_TX;
-            kout[k] = (zs - 1 >= z0 && ox <= Ly.xl && oy <= Ly.yl)
-                          ? kind_at(a.kind, static_cast<uint32_t>(Ly.off) + ox + static_cast<uint32_t>(Ly.pxy) * (zs - 1) +
-                                                static_cast<uint32_t>(Ly.px) * oy)
-                          : uint8_t(KIND_INERT);
-        }
         // (the empty asm keeps the Q plane bases from being hoisted, as in k_bulk2)
         int64_t plane = Ly.plane;
         asm volatile("" : "+s"(plane));
@@ -159,12 +160,20 @@ __global__ void __launch_bounds__(256) k_wall2(StepArgs a, int ntx, int nty, int
 #pragma unroll
         for (int k = 0; k < TB_PASS; ++k) {
             fl[k] = false;
-            wm[k] = 0;
+            if (k < TB_OUT) wm[k & (TB_OUT - 1)] = 0;
             if (!live[k]) continue;
-            const uint32_t e = (zs3[1] + rowb[k]) / TS;
-            const uint8_t kd = kcur[k];
+            const uint32_t eo = zs3[1] + py * static_cast<uint32_t>(gy[k]) + TS * static_cast<uint32_t>(gx[k]);
+            const uint32_t e = eo / TS;
+            const uint8_t kd = kind_at(a.kind, e);
             if (kd >= KIND_COND0) continue;
             fl[k] = true;
+            // a tile cell of the chunk next to non-fluid cells: its links
+            uint32_t lm = 0;
+            int lc = LCOND_MIXED;
+            if (k < TB_OUT && kd == KIND_NEAR && zs >= z0 && zs <= z1) {
+                lm = glob(a.lmask)[e];
+                lc = glob(a.lcond)[e];
+            }
             const uint32_t xs[3] = {TS * static_cast<uint32_t>(wrap_axis(gx[k] - 1, Ly.xl, Ly.per[0])),
                                     TS * static_cast<uint32_t>(gx[k]),
                                     TS * static_cast<uint32_t>(wrap_axis(gx[k] + 1, Ly.xl, Ly.per[0]))};
@@ -178,52 +187,113 @@ __global__ void __launch_bounds__(256) k_wall2(StepArgs a, int ntx, int nty, int
                 constexpr int ix = 1 - L::cx(q), iy = 1 - L::cy(q), iz = 1 - L::cz(q);
                 f[q] = IO::template load<q>(tb_at(src + q * plane, zs3[iz] + ys[iy] + xs[ix]));
             });
-            // links whose rule value step 2 will need from this cell
-            if (kd == KIND_NEAR && pi[k] >= 0 && pi[k] < TB_TX && pj[k] >= 0 && pj[k] < TB_TY)
-                wm[k] = glob(a.lmask)[e];
             collide_shifted<L, Calc, false, kRcp<M>, IO::SX>(f, static_cast<Calc>(a.omega), Calc(0), Calc(0), Calc(0),
                                                             Calc(0));
             static_for<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 const T v = IO::template store<q>(f[q]);
-                hold[k][q] = v;
                 if constexpr (L::cz(q) < 0) {
                     const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
                     if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY)
                         lds[S::template slot<q>()][dj * TB_TX + di] = v;
+                } else {
+                    hold[k][q] = v;
                 }
             });
+            if (k < TB_OUT && lm) {
+                // Step 2 of this cell pulls, for every link q of its mask, the
+                // rule value of the boundary cell b = x - c_q from this cell's
+                // step-1 values. They are evaluated now, while all Q values
+                // are in registers, and parked in the cell's own dst slots
+                // until phase 3 (which moves them to LDS). Nothing else reads
+                // those slots, and this thread overwrites them with the
+                // cell's step-2 result one plane later.
+                wm[k & (TB_OUT - 1)] = lm;
+                OwnAcc<L, Calc> acc;
+                static_for<0, Q>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    acc.template add<q>(IO::template to_calc<q>(IO::template store<q>(f[q])));
+                });
+                // Every value is computed before the first store (which the
+                // compiler cannot tell apart from the condition tables, so
+                // no load would pass it), and all Q slots are written (phase
+                // 3 moves only the mask's).
+                T rv[Q];
+                auto finv = [&](auto qc) {
+                    constexpr int qb = Q - 1 - decltype(qc)::value;
+                    return IO::template to_calc<qb>(IO::template store<qb>(f[qb]));
+                };
+                auto put = [&](auto qc, Calc v) {
+                    constexpr int q = decltype(qc)::value;
+                    rv[q] = IO::template store<q>(IO::template scaled<q>(v));
+                };
+                if (lc != LCOND_MIXED) {
+                    // One condition for every link: one branch on its kind,
+                    // then that family's loads go out together. The values
+                    // are rule_value's (same functions).
+                    const GCond& cd = cond_at(a.conds, lc);
+                    const int ck = cd.kind;
+                    if ((RULES & RULE_MOVINGWALL) != 0 && ck == BC_MOVINGWALL) {
+                        static_for<0, Q>([&](auto qc) {
+                            put(qc, moving_wall_rule<L, Calc, decltype(qc)::value>(acc.r, finv(qc), cd));
+                        });
+                    } else if ((RULES & RULE_INFLOW) != 0 && ck == BC_INFLOW) {
+                        static_for<0, Q>([&](auto qc) { put(qc, static_cast<Calc>(cd.geq[decltype(qc)::value])); });
+                    } else if ((RULES & RULE_OUTFLOW) != 0 && (ck == BC_OUTFLOW || ck == BC_PRESSURE)) {
+                        static_for<0, Q>([&](auto qc) {
+                            put(qc, outflow_rule<L, Calc, decltype(qc)::value>(acc.r, finv(qc), cd));
+                        });
+                    } else {  // NoSlip
+                        static_for<0, Q>([&](auto qc) { put(qc, finv(qc)); });
+                    }
+                } else {
+                    // links under several conditions (edges, corners): link
+                    // by link, each from its source's condition
+                    static_for<0, Q>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;  // pull direction from b = x - c_q
+                        rv[q] = T{};
+                        if (lm & (1u << q)) {
+                            const uint32_t be =
+                                (off + pz * static_cast<uint32_t>(wrap_axis(gz - L::cz(q), Ly.zl, Ly.per[2])) +
+                                 py * static_cast<uint32_t>(wrap_axis(gy[k] - L::cy(q), Ly.yl, Ly.per[1])) +
+                                 TS * static_cast<uint32_t>(wrap_axis(gx[k] - L::cx(q), Ly.xl, Ly.per[0]))) /
+                                TS;
+                            put(qc, rule_value<L, M, q, RULES>(cond_at(a.conds, kind_at(a.kind, be) - KIND_COND0),
+                                                               acc.r, finv(qc)));
+                        }
+                    });
+                }
+                static_for<0, Q>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    *tb_ptr(dst + q * plane, eo) = rv[q];
+                });
+            }
         }
         __syncthreads();
-        // (2) step 2 on plane zs - 1
+        // (2) step 2 on plane zs - 1: this thread's tile cells c = k * 256 + tid
         const int zo = zs - 1;
         if (zo >= z0) {
 #pragma unroll
             for (int k = 0; k < TB_OUT; ++k) {
+                if (!pfl[k]) continue;
                 const int c = k * 256 + tid;
-                const int ox = x0 + c % TB_TX, oy = y0 + c / TB_TX;
-                if (ox <= Ly.xl && oy <= Ly.yl) {
-                    const uint32_t eb = TS * (static_cast<uint32_t>(Ly.off) + ox) + pz * zo + py * oy;
-                    if (kout[k] < KIND_COND0) {
-                        Calc f[Q];
-                        static_for<0, Q>([&](auto qc) {
-                            constexpr int q = decltype(qc)::value;
-                            f[q] = IO::template load<q>(lds[S::template slot<q>()][c]);
-                        });
-                        collide_shifted<L, Calc, false, kRcp<M>, IO::SX>(f, static_cast<Calc>(a.omega), Calc(0),
-                                                                        Calc(0), Calc(0), Calc(0));
-                        static_for<0, Q>([&](auto qc) {
-                            constexpr int q = decltype(qc)::value;
-                            store_stream1<T>(tb_ptr(dst + q * plane, eb), IO::template store<q>(f[q]));
-                        });
-                    }
-                }
+                const uint32_t eb = TS * (static_cast<uint32_t>(Ly.off) + x0 + pi[k]) + pz * zo + py * (y0 + pj[k]);
+                Calc f[Q];
+                static_for<0, Q>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    f[q] = IO::template load<q>(lds[S::template slot<q>()][c]);
+                });
+                collide_shifted<L, Calc, false, kRcp<M>, IO::SX>(f, static_cast<Calc>(a.omega), Calc(0), Calc(0),
+                                                                Calc(0), Calc(0));
+                static_for<0, Q>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    store_stream1<T>(tb_ptr(dst + q * plane, eb), IO::template store<q>(f[q]));
+                });
             }
         }
         __syncthreads();
         // sends read by step 2 of the next iteration: c_z = 0 of this plane,
-        // c_z = +1 of the previous plane; then the rule values of this
-        // plane's fluid tile cells next to non-fluid cells
+        // c_z = +1 of the previous plane (held one more plane in registers)
 #pragma unroll
         for (int k = 0; k < TB_PASS; ++k) {
             if (!live[k]) continue;
@@ -239,28 +309,29 @@ __global__ void __launch_bounds__(256) k_wall2(StepArgs a, int ntx, int nty, int
             });
             pfl[k] = fl[k];
         }
+        // rule values parked in step 1 of this plane's tile cells: into the
+        // cells' own slots (written by no neighbour, since their senders are
+        // not fluid), read by step 2 in the next iteration
 #pragma unroll
-        for (int k = 0; k < TB_PASS; ++k) {
-            if (!wm[k]) continue;
-            OwnAcc<L, Calc> acc;
+        for (int k = 0; k < TB_OUT; ++k) {
+            const uint32_t lm = wm[k];
+            if (!lm) continue;
+            const uint32_t eo = zs3[1] + py * static_cast<uint32_t>(gy[k]) + TS * static_cast<uint32_t>(gx[k]);
+            const int c = k * 256 + tid;
+            // every load is issued before the first store (one round trip):
+            // the empty asm after the loads keeps them from sinking into
+            // the stores' branches
+            T pv[Q];
             static_for<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                acc.template add<q>(IO::template to_calc<q>(hold[k][q]));
+                pv[q] = T{};
+                if (lm & (1u << q)) pv[q] = tb_at(dst + q * plane, eo);
             });
-            const int cell = pj[k] * TB_TX + pi[k];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) asm volatile("" : "+v"(pv[q]));
             static_for<0, Q>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;  // pull direction from the boundary cell b = x - c_q
-                constexpr int qb = Q - 1 - q;
-                if (wm[k] & (1u << q)) {
-                    const uint32_t be =
-                        (off + pz * static_cast<uint32_t>(wrap_axis(gz - L::cz(q), Ly.zl, Ly.per[2])) +
-                         py * static_cast<uint32_t>(wrap_axis(gy[k] - L::cy(q), Ly.yl, Ly.per[1])) +
-                         TS * static_cast<uint32_t>(wrap_axis(gx[k] - L::cx(q), Ly.xl, Ly.per[0]))) /
-                        TS;
-                    const GCond& cd = cond_at(a.conds, kind_at(a.kind, be) - KIND_COND0);
-                    lds[S::template slot<q>()][cell] = IO::template store<q>(IO::template scaled<q>(
-                        rule_value<L, M, q, RULES>(cd, acc.r, IO::template to_calc<qb>(hold[k][qb]))));
-                }
+                constexpr int q = decltype(qc)::value;
+                if (lm & (1u << q)) lds[S::template slot<q>()][c] = pv[q];
             });
         }
     }
