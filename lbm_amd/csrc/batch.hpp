@@ -20,6 +20,7 @@ struct BatchView {
     void* buf[2];
     const uint8_t* kind;
     const uint32_t* lmask;
+    const uint8_t* lcond;
     const Cond* conds;
     double tau, omega;
     int bx_log2;  // row tiles: threads along x per tile = 1 << bx_log2 (of 256)

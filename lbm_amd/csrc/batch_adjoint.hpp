@@ -25,6 +25,7 @@ struct BatchAdjView {
     const void* gfinal;   // the forward run's final field (the domain's buffer)
     const uint8_t* kind;
     const uint32_t* lmask;
+    const uint8_t* lcond;
     const Cond* conds;
     double tau, omega;
     double* gw;           // 3 doubles per element: dJ/du_wall accumulated per cell

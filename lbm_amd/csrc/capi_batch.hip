@@ -86,6 +86,7 @@ int configure(lbm_batch* b, int V, int bx_policy, int bpc, int kernel) {
         v.buf[1] = d->buf[1];
         v.kind = d->d_kind;
         v.lmask = d->d_lmask;
+        v.lcond = d->d_lcond;
         v.conds = d->d_conds;
         v.tau = d->desc.tau;
         v.omega = 1.0 / d->desc.tau;
@@ -586,6 +587,7 @@ int lbm_batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_ever
         v.gfinal = d->buf[d->cur];
         v.kind = d->d_kind;
         v.lmask = d->d_lmask;
+        v.lcond = d->d_lcond;
         v.conds = d->d_conds;
         v.tau = d->desc.tau;
         v.omega = 1.0 / d->desc.tau;

@@ -80,7 +80,8 @@ struct StepArgs {
     // not (the links a near-boundary cell takes from boundary rules)
     const uint32_t* lmask;
     // per element: for a NEAR cell, the condition index shared by every link
-    // of its mask, or LCOND_MIXED where they differ (edges and corners)
+    // of its mask, or LCOND_MIXED where they differ (edges and corners);
+    // may be null (every cell then takes the per-link path)
     const uint8_t* lcond;
     const CellCoord* blist;  // cells of kind NEAR
     int64_t nb;

@@ -366,6 +366,17 @@ __device__ __forceinline__ OwnSums<typename Io<M, L>::Calc> own_sums_at(
     return acc.r;
 }
 
+// c ? a : b by bit masks. A plain select between two array elements can be
+// turned into a select between their addresses, which moves both arrays
+// out of registers into scratch memory.
+template <class T>
+__device__ __forceinline__ T pick_bits(bool c, T a, T b) {
+    using U = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
+    static_assert(sizeof(T) == sizeof(U), "float or double");
+    const U m = U(0) - static_cast<U>(c);
+    return __builtin_bit_cast(T, (__builtin_bit_cast(U, a) & m) | (__builtin_bit_cast(U, b) & ~m));
+}
+
 // The MovingWall and Outflow/Pressure values of the fp32/fp64 storage modes,
 // shared by rule_value and rule_value_flat so that both round alike.
 // MovingWall (include/boundary.hpp:55-65) with c_q . u_wall tabulated per
@@ -663,6 +674,39 @@ __device__ __forceinline__ bool any_cell(const StepArgs& a, int x, int y, int z)
             return IO::template to_calc<kk>(plane_ref<B32>(src, kk, Ly.plane, e));
         };
         const uint32_t lm = glob(a.lmask)[e];
+        if constexpr (M != Mode::EXACT64 && M != Mode::DUAL64) {
+            // Every link under one condition (not FreeSlip, whose rule reads
+            // other cells): one branch on its kind, then that family's loads
+            // go out together. The values are rule_value's (same functions).
+            // (A caller without the shared-condition table takes the path
+            // below.)
+            const int lc = a.lcond ? static_cast<int>(glob(a.lcond)[e]) : static_cast<int>(LCOND_MIXED);
+            if (lc != LCOND_MIXED) {
+                const GCond& cd = cond_at(a.conds, lc);
+                const int ck = cd.kind;
+                if (!FS || ck != BC_FREESLIP) {
+                    auto finv = [&](auto qc) { return own(std::integral_constant<int, Q - 1 - decltype(qc)::value>{}); };
+                    auto put = [&](auto qc, Calc v) {
+                        constexpr int q = decltype(qc)::value;
+                        f[q] = pick_bits(((lm >> q) & 1u) != 0, IO::template scaled<q>(v), f[q]);
+                    };
+                    if (ck == BC_MOVINGWALL) {
+                        static_for<0, Q>([&](auto qc) {
+                            put(qc, moving_wall_rule<L, Calc, decltype(qc)::value>(os, finv(qc), cd));
+                        });
+                    } else if (ck == BC_INFLOW) {
+                        static_for<0, Q>([&](auto qc) { put(qc, static_cast<Calc>(cd.geq[decltype(qc)::value])); });
+                    } else if (ck == BC_OUTFLOW || ck == BC_PRESSURE) {
+                        static_for<0, Q>([&](auto qc) {
+                            put(qc, outflow_rule<L, Calc, decltype(qc)::value>(os, finv(qc), cd));
+                        });
+                    } else {  // NoSlip
+                        static_for<0, Q>([&](auto qc) { put(qc, finv(qc)); });
+                    }
+                    goto collide;
+                }
+            }
+        }
         static_for<0, Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
@@ -679,6 +723,7 @@ __device__ __forceinline__ bool any_cell(const StepArgs& a, int x, int y, int z)
         });
     }
 
+collide:
     bool finite = true;
     if constexpr (M == Mode::EXACT64) {
         collide_exact<L>(f, a.tau);
@@ -699,10 +744,19 @@ __device__ __forceinline__ bool any_cell(const StepArgs& a, int x, int y, int z)
     return finite;
 }
 
+// Blocks of 256 per CU the cell-tile kernel is compiled for: 6 (80 VGPRs)
+// with 32- and 16-bit storage up to D3Q19; no bound with 64-bit arithmetic,
+// whose values take twice the registers, or with D3Q27, where 80 VGPRs
+// spill.
+template <class L, Mode M>
+constexpr int cells_min_blocks() {
+    return (M == Mode::EXACT64 || M == Mode::F64S || M == Mode::DUAL64 || L::Q > 19) ? 1 : 6;
+}
+
 // Cell-tile kernel: wavefront w of the grid takes the 64 cells
 // 64 (w mod tpp) .. + 63 (x fastest) of the z plane 1 + w / tpp.
 template <class L, Mode M, bool FS = true, bool B32 = false>
-__global__ void __launch_bounds__(256) k_cells(StepArgs a, int tpp, double rxl) {
+__global__ void __launch_bounds__(256, (cells_min_blocks<L, M>())) k_cells(StepArgs a, int tpp, double rxl) {
     const int64_t w = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int z = 1 + static_cast<int>(w / tpp);
     if (z > a.L.zl) return;

@@ -40,7 +40,10 @@ __device__ __forceinline__ int wrap_axis(int v, int l, int per) { return per ? w
 // One thread per near-boundary cell x (the domain's list): for every
 // direction q whose source b = x - c_q is not fluid, b[q] in src <- the rule
 // value of (b, q) from x's stored values. Reads fluid cells, writes
-// non-fluid cells only.
+// non-fluid cells only. x's Q values are loaded once, every value is
+// computed before the first store, and where all links share one condition
+// its data is loaded once, so a thread waits for memory about twice instead
+// of twice per link.
 template <class L, Mode M, int RULES>
 __global__ void __launch_bounds__(256) k_emit_rules(StepArgs a) {
     using IO = Io<M, L>;
@@ -54,17 +57,56 @@ __global__ void __launch_bounds__(256) k_emit_rules(StepArgs a) {
     T* __restrict__ buf = static_cast<T*>(a.dst);
     const uint32_t e = elem_at<true>(Ly, c.x, c.y, c.z);
     const uint32_t lm = glob(a.lmask)[e];
-    const OwnSums<Calc> os = own_sums_at<L, M, true>(buf, Ly.plane, e);
+    const int lc = glob(a.lcond)[e];
+    Calc own[Q];
+    OwnAcc<L, Calc> acc;
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int qb = Q - 1 - q;
+        own[q] = IO::template to_calc<q>(plane_ref<true>(buf, q, Ly.plane, e));
+        acc.template add<q>(own[q]);
+    });
+    T rv[Q];
+    auto put = [&](auto qc, Calc v) {
+        constexpr int q = decltype(qc)::value;
+        rv[q] = IO::template from_calc<q>(v);
+    };
+    auto finv = [&](auto qc) { return own[Q - 1 - decltype(qc)::value]; };
+    if (lc != LCOND_MIXED) {
+        const GCond& cd = cond_at(a.conds, lc);
+        const int ck = cd.kind;
+        if ((RULES & RULE_MOVINGWALL) != 0 && ck == BC_MOVINGWALL) {
+            static_for<0, Q>([&](auto qc) {
+                put(qc, moving_wall_rule<L, Calc, decltype(qc)::value>(acc.r, finv(qc), cd));
+            });
+        } else if ((RULES & RULE_INFLOW) != 0 && ck == BC_INFLOW) {
+            static_for<0, Q>([&](auto qc) { put(qc, static_cast<Calc>(cd.geq[decltype(qc)::value])); });
+        } else if ((RULES & RULE_OUTFLOW) != 0 && (ck == BC_OUTFLOW || ck == BC_PRESSURE)) {
+            static_for<0, Q>([&](auto qc) {
+                put(qc, outflow_rule<L, Calc, decltype(qc)::value>(acc.r, finv(qc), cd));
+            });
+        } else {  // NoSlip
+            static_for<0, Q>([&](auto qc) { put(qc, finv(qc)); });
+        }
+    } else {
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            rv[q] = T{};
+            if (lm & (1u << q)) {
+                const uint32_t se = elem_at<true>(Ly, wrap_if(c.x - L::cx(q), Ly.xl, Ly.per[0]),
+                                                  wrap_if(c.y - L::cy(q), Ly.yl, Ly.per[1]),
+                                                  wrap_if(c.z - L::cz(q), Ly.zl, Ly.per[2]));
+                put(qc, rule_value<L, M, q, RULES>(cond_at(a.conds, kind_at(a.kind, se) - KIND_COND0), acc.r,
+                                                   finv(qc)));
+            }
+        });
+    }
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
         if (lm & (1u << q)) {
             const uint32_t se = elem_at<true>(Ly, wrap_if(c.x - L::cx(q), Ly.xl, Ly.per[0]),
                                               wrap_if(c.y - L::cy(q), Ly.yl, Ly.per[1]),
                                               wrap_if(c.z - L::cz(q), Ly.zl, Ly.per[2]));
-            const Calc finv = IO::template to_calc<qb>(plane_ref<true>(buf, qb, Ly.plane, e));
-            plane_ref<true>(buf, q, Ly.plane, se) = IO::template from_calc<q>(
-                rule_value<L, M, q, RULES>(cond_at(a.conds, kind_at(a.kind, se) - KIND_COND0), os, finv));
+            plane_ref<true>(buf, q, Ly.plane, se) = rv[q];
         }
     });
 }

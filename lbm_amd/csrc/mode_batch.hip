@@ -13,6 +13,7 @@ __device__ __forceinline__ StepArgs view_args(const BatchView& v, int parity, in
     a.dst = v.buf[parity ^ 1];
     a.kind = v.kind;
     a.lmask = v.lmask;
+    a.lcond = v.lcond;
     a.conds = v.conds;
     a.materialised = mat;
     a.tau = v.tau;

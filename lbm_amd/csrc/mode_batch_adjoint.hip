@@ -58,6 +58,7 @@ __global__ void __launch_bounds__(256) k_badj_recompute(BatchAdjArgs b) {
         a.dst = slot<T>(v, b.dst_slot);
         a.kind = v.kind;
         a.lmask = v.lmask;
+        a.lcond = v.lcond;
         a.conds = v.conds;
         a.materialised = b.materialised;
         a.tau = v.tau;

@@ -8,8 +8,8 @@ lid-velocity gradient, fp32 against fp64 adjoints, and the full inverse
 problem: D3Q19 128^3, Brinkman porosity on a 32^3 grid upsampled
 trilinearly, body force 1e-5 along x, 2000 steps, sqrt(N) checkpoints,
 misfit of u on the mid-x plane against a second seeded porosity; 16 random
-coarse directions checked by central differences (eps = 1e-3) with an
-agreement bound of 1e-4 relative.
+coarse directions checked by central differences (eps = 1e-4) with an
+agreement bound of 1e-4 relative, direction by direction.
 """
 import sys
 
@@ -132,7 +132,13 @@ def test_c4_inverse_problem_gradient():
     """D3Q19 128^3 fp32, porosity from a seeded uniform 32^3 grid (trilinear,
     periodic), g = 1e-5 x, tau 0.8, 2000 steps, sqrt(N) checkpoints; target
     from porosity seed + 1; 16 random coarse directions by central FD
-    (eps = 1e-3) of the fp64 forward run; adjoint step <= 3x a forward step."""
+    (eps = 1e-4) of the fp64 forward run; adjoint step <= 3x a forward step.
+
+    eps: at 1e-3 the central difference's own O(eps^2) truncation error
+    reached 9.5e-5 of a direction's derivative (measured against the exact
+    fp64 JVP), the noise floor of a per-direction 1e-4 bound; at 1e-4 it
+    drops a hundredfold, while the rounding of the fp64 objective difference
+    stays near 1e-7 of the derivative."""
     n, nc, steps, tau = 128, 32, 2000, 0.8
     g = (1e-5, 0.0, 0.0)
     seed = 2024
@@ -168,7 +174,7 @@ def test_c4_inverse_problem_gradient():
     J64 = t64.forward(alpha)
     say(f"  J64 {J64:.6e}")
     rng = np.random.default_rng(seed + 2)
-    eps = 1e-3
+    eps = 1e-4
     adjs, fds, jvps = [], [], []
     for k in range(16):
         dc = rng.standard_normal(ac.shape)
@@ -182,14 +188,13 @@ def test_c4_inverse_problem_gradient():
             f"fd(fp64) {fds[-1]:+.8e}  rel vs jvp {rel(adjs[-1], jvps[-1]):.2e}  "
             f"fd-jvp rel {rel(fds[-1], jvps[-1]):.2e}")
     adjs, fds, jvps = map(np.array, (adjs, fds, jvps))
-    rms = float(np.sqrt((fds ** 2).mean()))
-    fd_err = float(np.abs(adjs - fds).max() / rms)
+    fd_err = float((np.abs(adjs - fds) / np.abs(fds)).max())
     jvp_err = float((np.abs(adjs - jvps) / np.abs(jvps)).max())
-    say(f"  max |adjoint - fd| / rms(fd) = {fd_err:.2e}; max per-direction |adjoint - jvp|/|jvp| = {jvp_err:.2e}")
-    # fp32 adjoint against the exact fp64 directional derivative, per direction
+    say(f"  max per-direction |adjoint - fd|/|fd| = {fd_err:.2e}; |adjoint - jvp|/|jvp| = {jvp_err:.2e}")
+    # fp32 adjoint against the exact fp64 directional derivative and against
+    # central differences (which carry an O(eps^2) truncation error of their
+    # own), each direction on its own
     assert jvp_err <= 1e-4
-    # against central differences (which carry an O(eps^2) truncation error of
-    # their own), relative to the size of the directional derivatives
     assert fd_err <= 1e-4
 
 

@@ -23,10 +23,13 @@ from lbm_amd import capi, workloads
 pytestmark = pytest.mark.gpu
 
 XL, YL, ZL, TAU, STEPS, U_IN = 384, 192, 192, 0.55, 5000, 0.04
-# Bounds sit above the measured errors (fp16 on the channel: 1.4e-3, its
-# velocities reach 0.12); bf16 (8-bit mantissa) gets a loose bound that
-# catches a broken mode, not a precision claim.
+# Regression guards, set above the measured errors: they catch a broken
+# mode, they are not the precision target. fp16 on the channel measures
+# about 1.4e-3 (its velocities reach 0.12) and misses the 1e-3 target,
+# which test_channel_f16_meets_target asserts as an expected failure; bf16
+# (8-bit mantissa) gets a loose bound.
 U_TOL = {"f32": 1e-5, "f16": 2e-3, "bf16": 3e-2}
+U_TARGET_F16 = 1e-3  # velocity relative L2 target for 16-bit storage
 U_TOL_TGV = {"f32": 1e-6, "f16": 1e-3, "bf16": 2e-2}
 
 
@@ -72,6 +75,17 @@ def test_channel_precision_vs_fp64(channel, storage):
         f"density deviation rel L2 {er[0]:.3e} max {er[1]:.3e}")
     assert np.isfinite(u[fluid]).all()
     assert eu[0] <= U_TOL[storage]
+
+
+@pytest.mark.xfail(reason="open: fp16 storage on the D3Q27 obstacle channel measures a velocity relative L2 "
+                          "of about 1.4e-3 against fp64, above the 1e-3 target (the Taylor-Green box meets it)",
+                   strict=False)
+def test_channel_f16_meets_target(channel):
+    sc, fluid, rho64, u64 = channel
+    rho, u = run("f16", sc)
+    eu = errors(u[fluid], u64[fluid])
+    say(f"  f16 against the 1e-3 target: velocity rel L2 {eu[0]:.3e}")
+    assert eu[0] <= U_TARGET_F16
 
 
 @pytest.fixture(scope="module")
