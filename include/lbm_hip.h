@@ -296,12 +296,25 @@ int lbm_batch_set_config(lbm_batch* b, int kernel, int vec, int bx_policy, int b
  * lockstep, objective J_d = kinetic energy of domain d after nsteps steps
  * from the domains' state at lbm_batch_adj_create, gradient of J_d with
  * respect to domain d's moving-wall velocity (summed over its MovingWall
- * conditions). Checkpoints every checkpoint_every steps (0: ceil(sqrt(N)));
- * each domain holds N/k + k + 1 field-sized slots on the device. Backward
- * recomputes each segment and steps the adjoint of all domains with one
- * launch per step. D3Q19, F32 or F64S, NoSlip / MovingWall / Inflow. */
+ * conditions). Checkpoints every checkpoint_every steps (k); each domain
+ * holds N/k + k + 1 field-sized slots on the device. Backward recomputes
+ * each segment and steps the adjoint of all domains with one launch per
+ * step. checkpoint_every = 0: k = ceil(sqrt(N)) where those slots fit in
+ * free device memory, else two levels as lbm_batch_adj_create2 with
+ * k = ceil(N^(2/3)) and sub_every = ceil(sqrt(k)). D3Q19, F32 or F64S,
+ * NoSlip / MovingWall / Inflow. */
 typedef struct lbm_batch_tape lbm_batch_tape;
 int lbm_batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_every, lbm_batch_tape** out);
+/* Two checkpoint levels: checkpoints every checkpoint_every steps (k) in the
+ * forward; the backward recomputes each segment once to lay down a
+ * checkpoint every sub_every steps (k2) inside it and differentiates the
+ * sub-segments from those. Slots per domain: N/k + k/k2 + k2 + 3, at the
+ * cost of one more recompute of every step. Gradients are the same bits as
+ * with one level. */
+int lbm_batch_adj_create2(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_every, uint64_t sub_every,
+                          lbm_batch_tape** out);
+/* out[0] = k, out[1] = k2 (0: one level), out[2] = slots per domain. */
+int lbm_batch_adj_levels(lbm_batch_tape* t, uint64_t out[3]);
 int lbm_batch_adj_destroy(lbm_batch_tape* t);
 /* J receives one objective value per domain (or NULL); leaves the final state
  * in the domains. */

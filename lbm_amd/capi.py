@@ -89,6 +89,8 @@ SIGNATURES = [
     ("lbm_batch_set_config", c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     ("lbm_batch_status", c_int, [c_void_p, POINTER(c_int), c_int]),
     ("lbm_batch_adj_create", c_int, [c_void_p, c_uint64, c_uint64, POINTER(c_void_p)]),
+    ("lbm_batch_adj_create2", c_int, [c_void_p, c_uint64, c_uint64, c_uint64, POINTER(c_void_p)]),
+    ("lbm_batch_adj_levels", c_int, [c_void_p, POINTER(c_uint64)]),
     ("lbm_batch_adj_destroy", c_int, [c_void_p]),
     ("lbm_batch_adj_forward", c_int, [c_void_p, POINTER(c_double)]),
     ("lbm_batch_adj_backward", c_int, [c_void_p, POINTER(c_double)]),
@@ -438,12 +440,23 @@ class BatchTape:
     """Batched adjoint over a Batch (lbm_batch_adj_*): forward() -> J per
     domain, backward() -> (ndomains, 3) lid-velocity gradients."""
 
-    def __init__(self, batch: Batch, nsteps: int, checkpoint_every: int = 0):
+    def __init__(self, batch: Batch, nsteps: int, checkpoint_every: int = 0, sub_every: int = 0):
+        """checkpoint_every = 0: automatic (sqrt(N), or two levels where
+        those slots do not fit); sub_every > 0: two levels, checkpoints every
+        checkpoint_every steps and, inside each segment, every sub_every."""
         self.batch = batch
         h = c_void_p()
-        _check(load().lbm_batch_adj_create(batch._h, nsteps, checkpoint_every, byref(h)))
+        if sub_every:
+            _check(load().lbm_batch_adj_create2(batch._h, nsteps, checkpoint_every, sub_every, byref(h)))
+        else:
+            _check(load().lbm_batch_adj_create(batch._h, nsteps, checkpoint_every, byref(h)))
         self._h = h
         self.n = len(batch.domains)
+
+    def levels(self) -> dict:
+        out = (ctypes.c_uint64 * 3)()
+        _check(load().lbm_batch_adj_levels(self._h, out))
+        return {"checkpoint_every": int(out[0]), "sub_every": int(out[1]), "slots_per_domain": int(out[2])}
 
     def close(self):
         if getattr(self, "_h", None):

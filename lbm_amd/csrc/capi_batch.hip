@@ -490,12 +490,19 @@ int lbm_batch_get_info(lbm_batch* b, lbm_batch_info* info) {
 // moving-wall velocity. Forward: the batch's graph-captured steps with a
 // checkpoint of every domain every k steps. Backward: per segment, one
 // recompute launch and one adjoint launch per step for all domains.
+//
+// Two levels (k2 > 0): the forward keeps a checkpoint every k steps; the
+// backward recomputes each such segment once to lay down a checkpoint
+// every k2 steps inside it, then treats each sub-segment as above. Slots
+// per domain: N/k + k/k2 + k2 + 3 instead of N/k + k + 1, for one more
+// recompute of every step.
 
 struct lbm_batch_tape {
     lbm_batch* b = nullptr;
     const BatchAdjOps* ops = nullptr;
     const AdjOps* aops = nullptr;
     uint64_t T = 0, k = 0, nck = 0;
+    uint64_t k2 = 0, n2 = 0;  // second level: sub-segment length (0: one level) and sub-segments per segment
     int nslots = 0;
     std::vector<void*> chunk;
     double* d_gw = nullptr;
@@ -524,9 +531,17 @@ void btape_free(lbm_batch_tape* t) {
     t->d_out = nullptr;
 }
 
-// slots: checkpoints 0 .. nck-1, segment states j = 1 .. k-1, two mu buffers
-int seg_slot(const lbm_batch_tape* t, uint64_t j) { return static_cast<int>(t->nck + j - 1); }
-int mu_slot(const lbm_batch_tape* t, int i) { return static_cast<int>(t->nck + t->k - 1) + i; }
+// Slots: checkpoints 0 .. nck-1; with two levels, sub-segment checkpoints
+// j = 1 .. n2-1 of the segment being differentiated; states j = 1 .. kin-1
+// of the (sub-)segment, kin = k2 or k; two mu buffers; with two levels, two
+// scratch states for laying down the sub-segment checkpoints.
+uint64_t inner_len(const lbm_batch_tape* t) { return t->k2 ? t->k2 : t->k; }
+int sub_slot(const lbm_batch_tape* t, uint64_t j) { return static_cast<int>(t->nck + j - 1); }
+int seg_slot(const lbm_batch_tape* t, uint64_t j) {
+    return static_cast<int>(t->nck + (t->k2 ? t->n2 - 1 : 0) + j - 1);
+}
+int mu_slot(const lbm_batch_tape* t, int i) { return seg_slot(t, inner_len(t)) + i; }
+int tmp_slot(const lbm_batch_tape* t, int i) { return mu_slot(t, 2) + i; }
 
 int grid_for(const lbm_batch_tape* t) {
     return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((t->ntiles + 3) / 4, 0x7fffffff)));
@@ -534,9 +549,9 @@ int grid_for(const lbm_batch_tape* t) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int lbm_batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_every, lbm_batch_tape** out) {
+int batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t k1, uint64_t k2, bool automatic, lbm_batch_tape** out) {
     if (!b || !out) return fail(LBM_ERR_ARG, "null argument");
     *out = nullptr;
     HIPCHK(hipSetDevice(b->device));
@@ -558,18 +573,38 @@ int lbm_batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_ever
     t->ops = ops;
     t->aops = aops;
     t->T = nsteps;
-    t->k = checkpoint_every ? checkpoint_every
-                            : static_cast<uint64_t>(std::ceil(std::sqrt(static_cast<double>(nsteps))));
-    if (t->k > t->T) t->k = t->T;
-    t->nck = (t->T + t->k - 1) / t->k;
-    t->nslots = static_cast<int>(t->nck + t->k + 1);
+    auto plan = [&](uint64_t ka, uint64_t kb) {
+        t->k = std::max<uint64_t>(1, std::min(ka, t->T));
+        t->nck = (t->T + t->k - 1) / t->k;
+        t->k2 = kb ? std::max<uint64_t>(1, std::min(kb, t->k)) : 0;
+        t->n2 = t->k2 ? (t->k + t->k2 - 1) / t->k2 : 0;
+        const uint64_t n = t->k2 ? t->nck + (t->n2 - 1) + (t->k2 - 1) + 2 + 2 : t->nck + (t->k - 1) + 2;
+        t->nslots = static_cast<int>(std::min<uint64_t>(n, 0x7fffffff));
+        return n;
+    };
+    size_t field_total = 0;
+    for (const lbm_domain* d : b->doms) field_total += d->field_bytes;
+    if (automatic) {
+        // sqrt(N) checkpoints where their slots fit in free memory (leaving a
+        // tenth of it), else two levels of about N^(2/3) and N^(1/3) steps
+        const uint64_t ks = static_cast<uint64_t>(std::ceil(std::sqrt(static_cast<double>(nsteps))));
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        const uint64_t n1 = plan(ks, 0);
+        if (static_cast<double>(n1) * static_cast<double>(field_total) > 0.9 * static_cast<double>(free_b)) {
+            const uint64_t ka = static_cast<uint64_t>(std::ceil(std::pow(static_cast<double>(nsteps), 2.0 / 3.0)));
+            plan(ka, static_cast<uint64_t>(std::ceil(std::sqrt(static_cast<double>(ka)))));
+        }
+    } else {
+        plan(k1, k2);
+    }
     t->start_mat = mat != 0;
     auto bail = [&](int code, const std::string& m) {
         btape_free(t);
         delete t;
         return fail(code, m);
     };
-    if (t->nck + t->k + 1 > 0x7fffffff) return bail(LBM_ERR_ARG, "too many checkpoint slots");
+    if (t->nslots >= 0x7fffffff) return bail(LBM_ERR_ARG, "too many checkpoint slots");
 
     std::vector<TileRef> tiles;
     int64_t gw_elems = 0;
@@ -578,7 +613,8 @@ int lbm_batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_ever
         lbm_domain* d = b->doms[di];
         void* c = nullptr;
         if (hipMalloc(&c, static_cast<size_t>(t->nslots) * d->field_bytes) != hipSuccess)
-            return bail(LBM_ERR_ALLOC, "checkpoint slots do not fit in device memory (lower nsteps or raise checkpoint_every)");
+            return bail(LBM_ERR_ALLOC,
+                        "checkpoint slots do not fit in device memory (lower nsteps, or use two checkpoint levels)");
         t->chunk.push_back(c);
         BatchAdjView v{};
         v.L = d->L;
@@ -614,6 +650,28 @@ int lbm_batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_ever
         hipStreamSynchronize(b->stream) != hipSuccess)
         return bail(LBM_ERR_HIP, "tape setup");
     *out = t;
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_adj_create(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_every, lbm_batch_tape** out) {
+    return batch_adj_create(b, nsteps, checkpoint_every, 0, checkpoint_every == 0, out);
+}
+
+int lbm_batch_adj_create2(lbm_batch* b, uint64_t nsteps, uint64_t checkpoint_every, uint64_t sub_every,
+                          lbm_batch_tape** out) {
+    if (checkpoint_every == 0 || sub_every == 0) return fail(LBM_ERR_ARG, "checkpoint_every and sub_every must be > 0");
+    return batch_adj_create(b, nsteps, checkpoint_every, sub_every, false, out);
+}
+
+int lbm_batch_adj_levels(lbm_batch_tape* t, uint64_t out[3]) {
+    if (!t || !out) return fail(LBM_ERR_ARG, "null argument");
+    out[0] = t->k;
+    out[1] = t->k2;
+    out[2] = static_cast<uint64_t>(t->nslots);
     return LBM_OK;
 }
 
@@ -692,40 +750,80 @@ int lbm_batch_adj_backward(lbm_batch_tape* t, double* grad_wall) {
     a.ntiles = t->ntiles;
     a.idx32 = b->all_idx32 ? 1 : 0;
     const int grid = grid_for(t);
-    Events ev(3 * t->nck);
+    Events ev(2);
     HIPCHK(ev.create());
+    double rec = 0, adj = 0;
+    // elapsed time between two points of the stream, into acc
+    auto timed = [&](double& acc, auto&& body) -> int {
+        HIPCHK(hipEventRecord(ev[0], b->stream));
+        if (int rc = body()) return rc;
+        HIPCHK(hipEventRecord(ev[1], b->stream));
+        HIPCHK(hipEventSynchronize(ev[1]));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        acc += ms;
+        return LBM_OK;
+    };
+    // recompute states start+1 .. end-1 from slot first (state `start`) into
+    // the segment slots, then the adjoint steps end .. start+1
+    auto segment = [&](uint64_t start, uint64_t end, int first) -> int {
+        auto state = [&](uint64_t s) { return s == start ? first : seg_slot(t, s - start); };
+        if (int rc = timed(rec, [&]() -> int {
+                for (uint64_t s = start + 1; s < end; ++s) {
+                    a.src_slot = state(s - 1);
+                    a.dst_slot = state(s);
+                    a.materialised = (s == 1 && t->start_mat) ? 1 : 0;
+                    HIPCHK(t->ops->recompute(a, grid, b->stream));
+                }
+                return LBM_OK;
+            }))
+            return rc;
+        return timed(adj, [&]() -> int {
+            for (uint64_t s = end; s > start; --s) {
+                a.gprev_slot = state(s - 1);
+                a.mu_in_slot = mu_slot(t, static_cast<int>((s + 1) & 1));
+                a.mu_out_slot = mu_slot(t, static_cast<int>(s & 1));
+                a.terminal = (s == t->T) ? 1 : 0;
+                a.materialised = (s == 1 && t->start_mat) ? 1 : 0;
+                HIPCHK(t->ops->backward(a, grid, b->stream));
+            }
+            return LBM_OK;
+        });
+    };
     for (uint64_t si = t->nck; si-- > 0;) {
         const uint64_t start = si * t->k;
         const uint64_t end = std::min(start + t->k, t->T);
-        auto state = [&](uint64_t s) { return s == start ? static_cast<int>(si) : seg_slot(t, s - start); };
-        HIPCHK(hipEventRecord(ev[3 * si + 0], b->stream));
-        for (uint64_t s = start + 1; s < end; ++s) {
-            a.src_slot = state(s - 1);
-            a.dst_slot = state(s);
-            a.materialised = (s == 1 && t->start_mat) ? 1 : 0;
-            HIPCHK(t->ops->recompute(a, grid, b->stream));
+        if (!t->k2) {
+            if (int rc = segment(start, end, static_cast<int>(si))) return rc;
+            continue;
         }
-        HIPCHK(hipEventRecord(ev[3 * si + 1], b->stream));
-        for (uint64_t s = end; s > start; --s) {
-            a.gprev_slot = state(s - 1);
-            a.mu_in_slot = mu_slot(t, static_cast<int>((s + 1) & 1));
-            a.mu_out_slot = mu_slot(t, static_cast<int>(s & 1));
-            a.terminal = (s == t->T) ? 1 : 0;
-            a.materialised = (s == 1 && t->start_mat) ? 1 : 0;
-            HIPCHK(t->ops->backward(a, grid, b->stream));
+        // second level: one recompute of the segment lays down the checkpoints
+        // of its sub-segments (states start + j k2), passing through two
+        // scratch slots in between
+        const uint64_t nsub = (end - start + t->k2 - 1) / t->k2;
+        if (int rc = timed(rec, [&]() -> int {
+                int src = static_cast<int>(si);
+                const uint64_t last = start + (nsub - 1) * t->k2;
+                for (uint64_t s = start + 1; s <= last; ++s) {
+                    const int dst = (s - start) % t->k2 == 0 ? sub_slot(t, (s - start) / t->k2)
+                                                              : tmp_slot(t, static_cast<int>(s & 1));
+                    a.src_slot = src;
+                    a.dst_slot = dst;
+                    a.materialised = (s == 1 && t->start_mat) ? 1 : 0;
+                    HIPCHK(t->ops->recompute(a, grid, b->stream));
+                    src = dst;
+                }
+                return LBM_OK;
+            }))
+            return rc;
+        for (uint64_t sj = nsub; sj-- > 0;) {
+            const uint64_t s0 = start + sj * t->k2;
+            const uint64_t s1 = std::min(s0 + t->k2, end);
+            if (int rc = segment(s0, s1, sj == 0 ? static_cast<int>(si) : sub_slot(t, sj))) return rc;
         }
-        HIPCHK(hipEventRecord(ev[3 * si + 2], b->stream));
     }
     HIPCHK(t->ops->reduce(t->d_views, static_cast<int>(t->views.size()), t->d_out, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    double rec = 0, adj = 0;
-    for (uint64_t si = 0; si < t->nck; ++si) {
-        float ms;
-        HIPCHK(hipEventElapsedTime(&ms, ev[3 * si], ev[3 * si + 1]));
-        rec += ms;
-        HIPCHK(hipEventElapsedTime(&ms, ev[3 * si + 1], ev[3 * si + 2]));
-        adj += ms;
-    }
     t->timing[0] = rec;
     t->timing[1] = adj;
     if (grad_wall)

@@ -150,3 +150,31 @@ def test_batch_adjoint_matches_single_tapes_and_differences():
         fd = (js[0] - js[1]) / (2 * eps)
         say(f"  lid component {comp}: adjoint {g64[0][comp]:.10e} fd {fd:.10e}")
         assert abs(g64[0][comp] - fd) <= 1e-6 * abs(fd)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64s"])
+def test_batch_adjoint_two_levels_equal_one_level(storage):
+    """Two checkpoint levels (checkpoints every 12 steps, every 5 inside a
+    segment) give the objective and the gradients of one level (every 7
+    steps) bit for bit, on 6 cavities over 40 steps."""
+    specs = workloads.cavity_set(n=6, lmin=10, lmax=20, re_min=50, re_max=200, seed=3)
+    steps, lid = 40, (workloads.U_LID, 0.01, 0.0)
+    out = []
+    for kw in ({"checkpoint_every": 7}, {"checkpoint_every": 12, "sub_every": 5}):
+        doms = [lid_cavity(*s[:4], storage, lid) for s in specs]
+        b = capi.Batch(doms)
+        t = capi.BatchTape(b, steps, **kw)
+        lv = t.levels()
+        J = t.forward()
+        g = t.backward()
+        say(f"  {storage} {kw}: levels {lv}, timing {t.timing()}")
+        out.append((lv, J, g))
+        t.close()
+        b.close()
+        for d in doms:
+            d.close()
+    (l1, J1, g1), (l2, J2, g2) = out
+    assert l1 == {"checkpoint_every": 7, "sub_every": 0, "slots_per_domain": 6 + 6 + 2}
+    assert l2 == {"checkpoint_every": 12, "sub_every": 5, "slots_per_domain": 4 + 2 + 4 + 2 + 2}
+    assert np.array_equal(J1, J2)
+    assert np.array_equal(g1.view(np.uint64), g2.view(np.uint64))
