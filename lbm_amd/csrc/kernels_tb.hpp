@@ -20,6 +20,13 @@
 // of the iteration before they are read, so LDS holds one plane of slots per
 // direction: Q x TX x TY values.
 //
+// With 16-bit storage the values travel in gather form: every patch cell
+// writes its values into its own slots (patch index p = (pj + 1) (TX + 2) +
+// pi + 1) with no test, and step 2 reads slot p(x - c_q) of direction q.
+// That takes (TX + 2)(TY + 2) slots per direction instead of TX TY, and
+// leaves out the per-direction bounds tests of the scatter form, which
+// weigh on the 16-bit pass (bound by instruction issue, not by memory).
+//
 // Coordinates are kept unwrapped (x0 - 1 .. x0 + TX, zs = z0 - 1 .. z1 + 1)
 // for the LDS bookkeeping and wrapped onto the periodic box only to address
 // global memory, which keeps boxes of one or two cells along an axis exact.
@@ -62,9 +69,17 @@ struct TbSlots {
     }
 };
 
+// LDS rows of one direction: with 16-bit storage one slot per patch cell
+// (the gather form below), else one per tile cell.
+template <class L, Mode M>
+struct TbCols {
+    static constexpr bool gather = sizeof(typename Io<M, L>::T) == 2;
+    static constexpr int n = gather ? TB_NP : TB_CELLS;
+};
+
 template <class L, Mode M>
 constexpr size_t tb_lds_bytes() {
-    return static_cast<size_t>(TbSlots<L>::NSLOT) * TB_CELLS * sizeof(typename Io<M, L>::T);
+    return static_cast<size_t>(TbSlots<L>::NSLOT) * TbCols<L, M>::n * sizeof(typename Io<M, L>::T);
 }
 
 // base[byte_off]: a uniform base and a 32-bit byte offset, the form of a
@@ -107,7 +122,8 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
     using Calc = typename IO::Calc;
     using S = TbSlots<L>;
     constexpr int Q = L::Q;
-    __shared__ T lds[S::NSLOT][TB_CELLS];
+    constexpr bool GATHER = TbCols<L, M>::gather;
+    __shared__ T lds[S::NSLOT][TbCols<L, M>::n];
 
     const FieldLayout& Ly = a.L;
     const int tid = threadIdx.x;
@@ -138,6 +154,18 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
         gx[k] = wrap1(ux, Ly.xl);
         gy[k] = wrap1(uy, Ly.yl);
     }
+
+    // where patch cell k of this thread sends value q (scatter form: the
+    // tile cell x + c_q, if in the tile; gather form: its own slot)
+    auto send = [&](auto qc, int k, T v) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (GATHER) {
+            lds[S::template slot<q>()][k * 256 + tid] = v;
+        } else {
+            const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
+            if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY) lds[S::template slot<q>()][dj * TB_TX + di] = v;
+        }
+    };
 
     T hold[TB_PASS][Q];
     // c_z = +1 values of the previous plane (only those are used); the first
@@ -175,9 +203,7 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                 constexpr int q = decltype(qc)::value;
                 const T v = IO::template store<q>(f[q]);
                 if constexpr (L::cz(q) < 0) {
-                    const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
-                    if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY)
-                        lds[S::template slot<q>()][dj * TB_TX + di] = v;
+                    send(qc, k, v);
                 } else {
                     hold[k][q] = v;
                 }
@@ -193,9 +219,13 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
                 const int ox = x0 + c % TB_TX, oy = y0 + c / TB_TX;
                 if (ox <= Ly.xl && oy <= Ly.yl) {
                     Calc f[Q];
+                    const int pc = (c / TB_TX + 1) * TB_PX + c % TB_TX + 1;  // c's patch index
                     static_for<0, Q>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
-                        f[q] = IO::template load<q>(lds[S::template slot<q>()][c]);
+                        if constexpr (GATHER)
+                            f[q] = IO::template load<q>(lds[S::template slot<q>()][pc - L::cx(q) - TB_PX * L::cy(q)]);
+                        else
+                            f[q] = IO::template load<q>(lds[S::template slot<q>()][c]);
                     });
                     const uint32_t eb = TS * (static_cast<uint32_t>(Ly.off) + ox) + pz * zo + py * oy;
                     Calc al = Calc(0);
@@ -216,11 +246,7 @@ __global__ void __launch_bounds__(256) k_bulk2(StepArgs a, int ntx, int nty, int
             if (!live[k]) continue;
             static_for<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                if constexpr (L::cz(q) >= 0) {
-                    const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
-                    if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY)
-                        lds[S::template slot<q>()][dj * TB_TX + di] = L::cz(q) == 0 ? hold[k][q] : prev[k][q];
-                }
+                if constexpr (L::cz(q) >= 0) send(qc, k, L::cz(q) == 0 ? hold[k][q] : prev[k][q]);
                 if constexpr (L::cz(q) > 0) prev[k][q] = hold[k][q];
             });
         }
