@@ -17,10 +17,13 @@
 //   - non-fluid cells (ghost layer, solids) do neither step and send nothing;
 //   - step 2 needs the rule values of x's non-fluid sources b = x - c_k from
 //     x's own step-1 values. No neighbour sends those (b is not fluid), so
-//     after step 1 the thread that computed x evaluates, for every link k of
-//     x's mask (lmask: bit k when x - c_k is not fluid), the rule value of
-//     (b, k) and writes it into x's own slot k. Nothing else writes that
-//     slot: its only regular sender would be b.
+//     the thread that computed x keeps all of x's step-1 values in
+//     registers (the tile cells of a thread are the cells it also finishes
+//     in step 2) and, once step 2 of the plane before is done, evaluates for
+//     every link k of x's mask (lmask: bit k when x - c_k is not fluid) the
+//     rule value of (b, k) and writes it into x's own slot k. Nothing else
+//     writes that slot: its only regular sender would be b. Where all links
+//     share one condition (lcond) its data is loaded once for all of them.
 // The storage format is fp32, the arithmetic type, so a rule value written
 // to memory or LDS is the value the single step computes, and a pass equals
 // two single steps bit for bit on every fluid cell.
@@ -111,16 +114,8 @@ __global__ void __launch_bounds__(256) k_emit_rules(StepArgs a) {
     });
 }
 
-// Waves per SIMD the pass is compiled for: as many blocks as LDS holds
-// (160 KiB per CU, one wave of each block per SIMD), at most 4 (128 VGPRs).
-template <class L, Mode M>
-constexpr int wall2_waves() {
-    const size_t by_lds = (size_t{160} << 10) / tb_lds_bytes<L, M>();
-    return by_lds < 1 ? 1 : (by_lds > 4 ? 4 : static_cast<int>(by_lds));
-}
-
 template <class L, Mode M, int RULES>
-__global__ void __launch_bounds__(256, (wall2_waves<L, M>())) k_wall2(StepArgs a, int ntx, int nty, int zc, int nxcd) {
+__global__ void __launch_bounds__(256) k_wall2(StepArgs a, int ntx, int nty, int zc, int nxcd) {
     using IO = Io<M, L>;
     using T = typename IO::T;
     using Calc = typename IO::Calc;
@@ -180,10 +175,11 @@ __global__ void __launch_bounds__(256, (wall2_waves<L, M>())) k_wall2(StepArgs a
         gy[k] = wrap_axis(uy, Ly.yl, Ly.per[1]);
     }
 
-    T hold[TB_PASS][Q];  // step-1 values (c_z >= 0) of this plane's patch cells
+    T hold[TB_PASS][Q];  // step-1 values of this plane's patch cells (ring cells: c_z >= 0 only)
     T prev[TB_PASS][Q];  // the c_z = +1 values of the previous plane's
     bool fl[TB_PASS], pfl[TB_PASS];     // patch cell fluid on this / the previous plane
     uint32_t wm[TB_OUT];  // link masks of this plane's tile cells next to non-fluid cells
+    int lcs[TB_OUT];      // their shared condition
 #pragma unroll
     for (int k = 0; k < TB_PASS; ++k) {
         pfl[k] = false;
@@ -204,8 +200,7 @@ __global__ void __launch_bounds__(256, (wall2_waves<L, M>())) k_wall2(StepArgs a
             fl[k] = false;
             if (k < TB_OUT) wm[k & (TB_OUT - 1)] = 0;
             if (!live[k]) continue;
-            const uint32_t eo = zs3[1] + py * static_cast<uint32_t>(gy[k]) + TS * static_cast<uint32_t>(gx[k]);
-            const uint32_t e = eo / TS;
+            const uint32_t e = (zs3[1] + py * static_cast<uint32_t>(gy[k]) + TS * static_cast<uint32_t>(gx[k])) / TS;
             const uint8_t kd = kind_at(a.kind, e);
             if (kd >= KIND_COND0) continue;
             fl[k] = true;
@@ -238,77 +233,14 @@ __global__ void __launch_bounds__(256, (wall2_waves<L, M>())) k_wall2(StepArgs a
                     const int di = pi[k] + L::cx(q), dj = pj[k] + L::cy(q);
                     if (di >= 0 && di < TB_TX && dj >= 0 && dj < TB_TY)
                         lds[S::template slot<q>()][dj * TB_TX + di] = v;
-                } else {
-                    hold[k][q] = v;
                 }
+                // tile cells keep every value (their rule values are evaluated
+                // in phase 3), ring cells the ones sent later
+                if (k < TB_OUT || L::cz(q) >= 0) hold[k][q] = v;
             });
-            if (k < TB_OUT && lm) {
-                // Step 2 of this cell pulls, for every link q of its mask, the
-                // rule value of the boundary cell b = x - c_q from this cell's
-                // step-1 values. They are evaluated now, while all Q values
-                // are in registers, and parked in the cell's own dst slots
-                // until phase 3 (which moves them to LDS). Nothing else reads
-                // those slots, and this thread overwrites them with the
-                // cell's step-2 result one plane later.
+            if (k < TB_OUT) {
                 wm[k & (TB_OUT - 1)] = lm;
-                OwnAcc<L, Calc> acc;
-                static_for<0, Q>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    acc.template add<q>(IO::template to_calc<q>(IO::template store<q>(f[q])));
-                });
-                // Every value is computed before the first store (which the
-                // compiler cannot tell apart from the condition tables, so
-                // no load would pass it), and all Q slots are written (phase
-                // 3 moves only the mask's).
-                T rv[Q];
-                auto finv = [&](auto qc) {
-                    constexpr int qb = Q - 1 - decltype(qc)::value;
-                    return IO::template to_calc<qb>(IO::template store<qb>(f[qb]));
-                };
-                auto put = [&](auto qc, Calc v) {
-                    constexpr int q = decltype(qc)::value;
-                    rv[q] = IO::template store<q>(IO::template scaled<q>(v));
-                };
-                if (lc != LCOND_MIXED) {
-                    // One condition for every link: one branch on its kind,
-                    // then that family's loads go out together. The values
-                    // are rule_value's (same functions).
-                    const GCond& cd = cond_at(a.conds, lc);
-                    const int ck = cd.kind;
-                    if ((RULES & RULE_MOVINGWALL) != 0 && ck == BC_MOVINGWALL) {
-                        static_for<0, Q>([&](auto qc) {
-                            put(qc, moving_wall_rule<L, Calc, decltype(qc)::value>(acc.r, finv(qc), cd));
-                        });
-                    } else if ((RULES & RULE_INFLOW) != 0 && ck == BC_INFLOW) {
-                        static_for<0, Q>([&](auto qc) { put(qc, static_cast<Calc>(cd.geq[decltype(qc)::value])); });
-                    } else if ((RULES & RULE_OUTFLOW) != 0 && (ck == BC_OUTFLOW || ck == BC_PRESSURE)) {
-                        static_for<0, Q>([&](auto qc) {
-                            put(qc, outflow_rule<L, Calc, decltype(qc)::value>(acc.r, finv(qc), cd));
-                        });
-                    } else {  // NoSlip
-                        static_for<0, Q>([&](auto qc) { put(qc, finv(qc)); });
-                    }
-                } else {
-                    // links under several conditions (edges, corners): link
-                    // by link, each from its source's condition
-                    static_for<0, Q>([&](auto qc) {
-                        constexpr int q = decltype(qc)::value;  // pull direction from b = x - c_q
-                        rv[q] = T{};
-                        if (lm & (1u << q)) {
-                            const uint32_t be =
-                                (off + pz * static_cast<uint32_t>(wrap_axis(gz - L::cz(q), Ly.zl, Ly.per[2])) +
-                                 py * static_cast<uint32_t>(wrap_axis(gy[k] - L::cy(q), Ly.yl, Ly.per[1])) +
-                                 TS * static_cast<uint32_t>(wrap_axis(gx[k] - L::cx(q), Ly.xl, Ly.per[0]))) /
-                                TS;
-                            put(qc, rule_value<L, M, q, RULES>(cond_at(a.conds, kind_at(a.kind, be) - KIND_COND0),
-                                                               acc.r, finv(qc)));
-                        }
-                    });
-                }
-                static_for<0, Q>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    *tb_ptr(dst + q * plane, eo) = rv[q];
-                });
+                lcs[k & (TB_OUT - 1)] = lc;
             }
         }
         __syncthreads();
@@ -351,29 +283,63 @@ __global__ void __launch_bounds__(256, (wall2_waves<L, M>())) k_wall2(StepArgs a
             });
             pfl[k] = fl[k];
         }
-        // rule values parked in step 1 of this plane's tile cells: into the
-        // cells' own slots (written by no neighbour, since their senders are
-        // not fluid), read by step 2 in the next iteration
+        // rule values of this plane's tile cells next to non-fluid cells,
+        // from their step-1 values, into their own slots (written by no
+        // neighbour: the senders are not fluid); read by step 2 next
 #pragma unroll
         for (int k = 0; k < TB_OUT; ++k) {
             const uint32_t lm = wm[k];
             if (!lm) continue;
-            const uint32_t eo = zs3[1] + py * static_cast<uint32_t>(gy[k]) + TS * static_cast<uint32_t>(gx[k]);
             const int c = k * 256 + tid;
-            // every load is issued before the first store (one round trip):
-            // the empty asm after the loads keeps them from sinking into
-            // the stores' branches
-            T pv[Q];
+            OwnAcc<L, Calc> acc;
             static_for<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                pv[q] = T{};
-                if (lm & (1u << q)) pv[q] = tb_at(dst + q * plane, eo);
+                acc.template add<q>(IO::template to_calc<q>(hold[k][q]));
             });
-#pragma unroll
-            for (int q = 0; q < Q; ++q) asm volatile("" : "+v"(pv[q]));
+            auto finv = [&](auto qc) {
+                constexpr int qb = Q - 1 - decltype(qc)::value;
+                return IO::template to_calc<qb>(hold[k][qb]);
+            };
+            T rv[Q];
+            auto put = [&](auto qc, Calc v) {
+                constexpr int q = decltype(qc)::value;
+                rv[q] = IO::template store<q>(IO::template scaled<q>(v));
+            };
+            const int lc = lcs[k];
+            if (lc != LCOND_MIXED) {
+                const GCond& cd = cond_at(a.conds, lc);
+                const int ck = cd.kind;
+                if ((RULES & RULE_MOVINGWALL) != 0 && ck == BC_MOVINGWALL) {
+                    static_for<0, Q>([&](auto qc) {
+                        put(qc, moving_wall_rule<L, Calc, decltype(qc)::value>(acc.r, finv(qc), cd));
+                    });
+                } else if ((RULES & RULE_INFLOW) != 0 && ck == BC_INFLOW) {
+                    static_for<0, Q>([&](auto qc) { put(qc, static_cast<Calc>(cd.geq[decltype(qc)::value])); });
+                } else if ((RULES & RULE_OUTFLOW) != 0 && (ck == BC_OUTFLOW || ck == BC_PRESSURE)) {
+                    static_for<0, Q>([&](auto qc) {
+                        put(qc, outflow_rule<L, Calc, decltype(qc)::value>(acc.r, finv(qc), cd));
+                    });
+                } else {  // NoSlip
+                    static_for<0, Q>([&](auto qc) { put(qc, finv(qc)); });
+                }
+            } else {
+                static_for<0, Q>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;  // pull direction from b = x - c_q
+                    rv[q] = T{};
+                    if (lm & (1u << q)) {
+                        const uint32_t be =
+                            (off + pz * static_cast<uint32_t>(wrap_axis(gz - L::cz(q), Ly.zl, Ly.per[2])) +
+                             py * static_cast<uint32_t>(wrap_axis(gy[k] - L::cy(q), Ly.yl, Ly.per[1])) +
+                             TS * static_cast<uint32_t>(wrap_axis(gx[k] - L::cx(q), Ly.xl, Ly.per[0]))) /
+                            TS;
+                        put(qc, rule_value<L, M, q, RULES>(cond_at(a.conds, kind_at(a.kind, be) - KIND_COND0),
+                                                           acc.r, finv(qc)));
+                    }
+                });
+            }
             static_for<0, Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                if (lm & (1u << q)) lds[S::template slot<q>()][c] = pv[q];
+                if (lm & (1u << q)) lds[S::template slot<q>()][c] = rv[q];
             });
         }
     }
