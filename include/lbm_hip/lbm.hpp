@@ -73,9 +73,10 @@ struct DeviceOptions {
     int device = 0;
     std::array<int, 3> periodic{{0, 0, 0}};
     int layout = LBM_LAYOUT_AB;  // LBM_LAYOUT_AA: one buffer updated in place
-    // 2: where the domain allows it (bulk-only, fp32/fp16/bf16), step(n) runs
-    // pairs of steps as one pass over the field, bit-identical to single steps
-    int steps_per_pass = 2;
+    // lbm_set_time_blocking's steps_per_pass: 0 (automatic) runs pairs of
+    // steps as one pass over the field where that measured faster, 2 wherever
+    // the domain allows it, 1 never; bit-identical to single steps
+    int steps_per_pass = 0;
 };
 
 template <class M>
