@@ -627,6 +627,74 @@ __device__ __forceinline__ bool near_cell(const StepArgs& a, int x, int y, int z
 // equal those of bulk_cells / near_cell for the same cell. B32 takes the
 // cell's own and plain-pull addresses as 32-bit byte offsets (planes under
 // 4 GiB). Returns false when a fluid cell read a NaN or infinite value.
+// The pulls of a NEAR cell x (element e, ghosted-box coordinates x, y, z)
+// from non-fluid sources, f[q] for every link q of its mask, replaced by
+// their boundary values, from x's stored values in a.src (the reference's
+// boundary pass folded into the pull).
+template <class L, Mode M, bool FS, bool B32>
+__device__ __forceinline__ void near_rules(const StepArgs& a, typename ElemIdx<B32>::type e, int x, int y, int z,
+                                           typename Io<M, L>::Calc (&f)[L::Q]) {
+    using IO = Io<M, L>;
+    using T = typename IO::T;
+    using Calc = typename IO::Calc;
+    using I = typename ElemIdx<B32>::type;
+    constexpr int Q = L::Q;
+    const FieldLayout& Ly = a.L;
+    const T* __restrict__ src = static_cast<const T*>(a.src);
+    const OwnSums<Calc> os = own_sums_at<L, M, B32>(src, Ly.plane, e);
+    auto own = [&](auto kc) {
+        constexpr int kk = decltype(kc)::value;
+        return IO::template to_calc<kk>(plane_ref<B32>(src, kk, Ly.plane, e));
+    };
+    const uint32_t lm = glob(a.lmask)[e];
+    if constexpr (M != Mode::EXACT64 && M != Mode::DUAL64) {
+        // Every link under one condition (not FreeSlip, whose rule reads
+        // other cells): one branch on its kind, then that family's loads go
+        // out together. The values are rule_value's (same functions). (A
+        // caller without the shared-condition table takes the path below.)
+        const int lc = a.lcond ? static_cast<int>(glob(a.lcond)[e]) : static_cast<int>(LCOND_MIXED);
+        if (lc != LCOND_MIXED) {
+            const GCond& cd = cond_at(a.conds, lc);
+            const int ck = cd.kind;
+            if (!FS || ck != BC_FREESLIP) {
+                auto finv = [&](auto qc) { return own(std::integral_constant<int, Q - 1 - decltype(qc)::value>{}); };
+                auto put = [&](auto qc, Calc v) {
+                    constexpr int q = decltype(qc)::value;
+                    f[q] = pick_bits(((lm >> q) & 1u) != 0, IO::template scaled<q>(v), f[q]);
+                };
+                if (ck == BC_MOVINGWALL) {
+                    static_for<0, Q>([&](auto qc) {
+                        put(qc, moving_wall_rule<L, Calc, decltype(qc)::value>(os, finv(qc), cd));
+                    });
+                } else if (ck == BC_INFLOW) {
+                    static_for<0, Q>([&](auto qc) { put(qc, static_cast<Calc>(cd.geq[decltype(qc)::value])); });
+                } else if (ck == BC_OUTFLOW || ck == BC_PRESSURE) {
+                    static_for<0, Q>([&](auto qc) {
+                        put(qc, outflow_rule<L, Calc, decltype(qc)::value>(os, finv(qc), cd));
+                    });
+                } else {  // NoSlip
+                    static_for<0, Q>([&](auto qc) { put(qc, finv(qc)); });
+                }
+                return;
+            }
+        }
+    }
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
+        if (lm & (1u << q)) {
+            const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
+            const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
+            const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
+            const I se = elem_at<B32>(Ly, sx, sy, sz);
+            const uint8_t sk = kind_at(a.kind, se);
+            bool stale;
+            f[q] = IO::template scaled<q>(pull_boundary<L, M, q, FS, B32>(a, cond_at(a.conds, sk - KIND_COND0), os, own,
+                                                                           x, y, z, sx, sy, sz, se, stale));
+        }
+    });
+}
+
 template <class L, Mode M, bool FS = true, bool B32 = false>
 __device__ __forceinline__ bool any_cell(const StepArgs& a, int x, int y, int z) {
     using IO = Io<M, L>;
@@ -667,63 +735,8 @@ __device__ __forceinline__ bool any_cell(const StepArgs& a, int x, int y, int z)
         });
     }
     if (k >= KIND_COND0) return true;
-    if (k == KIND_NEAR && !a.materialised) {
-        const OwnSums<Calc> os = own_sums_at<L, M, B32>(src, Ly.plane, e);
-        auto own = [&](auto kc) {
-            constexpr int kk = decltype(kc)::value;
-            return IO::template to_calc<kk>(plane_ref<B32>(src, kk, Ly.plane, e));
-        };
-        const uint32_t lm = glob(a.lmask)[e];
-        if constexpr (M != Mode::EXACT64 && M != Mode::DUAL64) {
-            // Every link under one condition (not FreeSlip, whose rule reads
-            // other cells): one branch on its kind, then that family's loads
-            // go out together. The values are rule_value's (same functions).
-            // (A caller without the shared-condition table takes the path
-            // below.)
-            const int lc = a.lcond ? static_cast<int>(glob(a.lcond)[e]) : static_cast<int>(LCOND_MIXED);
-            if (lc != LCOND_MIXED) {
-                const GCond& cd = cond_at(a.conds, lc);
-                const int ck = cd.kind;
-                if (!FS || ck != BC_FREESLIP) {
-                    auto finv = [&](auto qc) { return own(std::integral_constant<int, Q - 1 - decltype(qc)::value>{}); };
-                    auto put = [&](auto qc, Calc v) {
-                        constexpr int q = decltype(qc)::value;
-                        f[q] = pick_bits(((lm >> q) & 1u) != 0, IO::template scaled<q>(v), f[q]);
-                    };
-                    if (ck == BC_MOVINGWALL) {
-                        static_for<0, Q>([&](auto qc) {
-                            put(qc, moving_wall_rule<L, Calc, decltype(qc)::value>(os, finv(qc), cd));
-                        });
-                    } else if (ck == BC_INFLOW) {
-                        static_for<0, Q>([&](auto qc) { put(qc, static_cast<Calc>(cd.geq[decltype(qc)::value])); });
-                    } else if (ck == BC_OUTFLOW || ck == BC_PRESSURE) {
-                        static_for<0, Q>([&](auto qc) {
-                            put(qc, outflow_rule<L, Calc, decltype(qc)::value>(os, finv(qc), cd));
-                        });
-                    } else {  // NoSlip
-                        static_for<0, Q>([&](auto qc) { put(qc, finv(qc)); });
-                    }
-                    goto collide;
-                }
-            }
-        }
-        static_for<0, Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int cx = L::cx(q), cy = L::cy(q), cz = L::cz(q);
-            if (lm & (1u << q)) {
-                const int sx = wrap_if(x - cx, Ly.xl, Ly.per[0]);
-                const int sy = wrap_if(y - cy, Ly.yl, Ly.per[1]);
-                const int sz = wrap_if(z - cz, Ly.zl, Ly.per[2]);
-                const I se = elem_at<B32>(Ly, sx, sy, sz);
-                const uint8_t sk = kind_at(a.kind, se);
-                bool stale;
-                f[q] = IO::template scaled<q>(pull_boundary<L, M, q, FS, B32>(a, cond_at(a.conds, sk - KIND_COND0), os,
-                                                                               own, x, y, z, sx, sy, sz, se, stale));
-            }
-        });
-    }
+    if (k == KIND_NEAR && !a.materialised) near_rules<L, M, FS, B32>(a, e, x, y, z, f);
 
-collide:
     bool finite = true;
     if constexpr (M == Mode::EXACT64) {
         collide_exact<L>(f, a.tau);
