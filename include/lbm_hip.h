@@ -221,6 +221,13 @@ int lbm_adj_forward(lbm_tape* t, const double* alpha, double* J);
 /* Backward after a forward: grad_alpha (xl*yl*zl, or NULL) and grad_wall[3]
  * (or NULL) receive dJ times the gradients. */
 int lbm_adj_backward(lbm_tape* t, double dJ, double* grad_alpha, double grad_wall[3]);
+/* lbm_adj_forward / lbm_adj_backward with the porosity and its gradient in
+ * device memory of the domain's GPU (xl*yl*zl doubles, x fastest), for
+ * callers whose fields already live there (torch tensors): no host copy.
+ * The caller has finished writing d_alpha before the call; the call returns
+ * with d_grad_alpha written. No reference counterpart. */
+int lbm_adj_forward_device(lbm_tape* t, const double* d_alpha, double* J);
+int lbm_adj_backward_device(lbm_tape* t, double dJ, double* d_grad_alpha, double grad_wall[3]);
 /* fp64 forward-mode directional derivative along dalpha (dual numbers through
  * the same step kernels): J and dJ/ds of J(alpha + s dalpha) at s = 0. */
 int lbm_jvp(lbm_tape* t, const double* alpha, const double* dalpha, double* J, double* dJ);

@@ -3,7 +3,9 @@ porosity map of the inverse problem.
 
 `LbmObjective.apply(alpha, tape)` runs lbm_adj_forward on a fine porosity
 field (torch tensor, (zl, yl, xl), float64) and returns J as a 0-d tensor;
-its backward calls lbm_adj_backward. `upsample_periodic` is the trilinear,
+its backward calls lbm_adj_backward. A porosity on the domain's GPU goes
+through lbm_adj_forward_device / lbm_adj_backward_device and never visits
+the host. `upsample_periodic` is the trilinear,
 periodic coarse -> fine map as a torch expression, so torch differentiates
 the composition (gradient with respect to the coarse grid) by itself.
 """
@@ -68,11 +70,19 @@ def make_objective_function():
             ctx.tape = tape
             ctx.shape = alpha.shape
             ctx.dtype, ctx.device = alpha.dtype, alpha.device
-            j = tape.forward(alpha.detach().cpu().double().numpy())
+            # a porosity already on the domain's GPU stays there
+            ctx.on_device = alpha.is_cuda and alpha.device.index == tape.dom.device
+            if ctx.on_device:
+                j = tape.forward_device(alpha.detach().to(torch.float64).contiguous())
+            else:
+                j = tape.forward(alpha.detach().cpu().double().numpy())
             return torch.tensor(j, dtype=alpha.dtype, device=alpha.device)
 
         @staticmethod
         def backward(ctx, grad_out):
+            if ctx.on_device:
+                g, _ = ctx.tape.backward_device(float(grad_out))
+                return g.to(ctx.dtype).reshape(ctx.shape), None
             g, _ = ctx.tape.backward(float(grad_out))
             return torch.as_tensor(g, dtype=ctx.dtype, device=ctx.device).reshape(ctx.shape), None
 

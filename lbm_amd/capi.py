@@ -88,6 +88,8 @@ SIGNATURES = [
     ("lbm_batch_get_info", c_int, [c_void_p, c_void_p]),
     ("lbm_batch_set_config", c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     ("lbm_batch_status", c_int, [c_void_p, POINTER(c_int), c_int]),
+    ("lbm_adj_forward_device", c_int, [c_void_p, c_void_p, POINTER(c_double)]),
+    ("lbm_adj_backward_device", c_int, [c_void_p, c_double, c_void_p, POINTER(c_double)]),
     ("lbm_batch_adj_create", c_int, [c_void_p, c_uint64, c_uint64, POINTER(c_void_p)]),
     ("lbm_batch_adj_create2", c_int, [c_void_p, c_uint64, c_uint64, c_uint64, POINTER(c_void_p)]),
     ("lbm_batch_adj_levels", c_int, [c_void_p, POINTER(c_uint64)]),
@@ -104,12 +106,33 @@ SIGNATURES = [
 _lib = None
 
 
+def _bind_torch_runtime() -> None:
+    """Loads torch's HIP runtime first, where torch is installed (without
+    importing torch). torch ships its own libamdhip64 (soname
+    libamdhip64.so.7) and asks for it by the name libamdhip64.so. Loaded
+    after this library, which links /opt/rocm's, it would be a second HIP
+    runtime in the process, under which torch finds no GPU. Loaded first,
+    this library binds to it, torch reuses it when imported, and device
+    memory passes between the two (lbm_adj_*_device with torch tensors).
+    LBM_HIP_NO_TORCH=1 keeps /opt/rocm's runtime."""
+    if os.environ.get("LBM_HIP_NO_TORCH"):
+        return
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin:
+        return
+    hip = Path(spec.origin).parent / "lib" / "libamdhip64.so"
+    if hip.exists():
+        ctypes.CDLL(str(hip), mode=ctypes.RTLD_GLOBAL)
+
+
 def load() -> ctypes.CDLL:
     """Loads liblbm_hip.so; raises if it has not been built."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m lbm_amd.build`")
+        _bind_torch_runtime()
         lib = ctypes.CDLL(str(LIB_PATH))
         for name, res, args in SIGNATURES:
             fn = getattr(lib, name)
@@ -182,6 +205,7 @@ class Domain:
         self._h = h
         self.q, self.xl, self.yl, self.zl, self.tau = q, xl, yl, zl, tau
         self.storage = storage
+        self.device = device
         self.periodic = tuple(bool(p) for p in periodic)
 
     # -- lifecycle -----------------------------------------------------------
@@ -357,6 +381,32 @@ class Tape:
         gw = np.zeros(3)
         _check(load().lbm_adj_backward(self._h, dJ, _dptr(g) if want_alpha else None, _dptr(gw)))
         return g, gw
+
+    def forward_device(self, alpha) -> float:
+        """forward() with a porosity torch tensor on the domain's GPU (float64,
+        (zl, yl, xl), contiguous); no host copy."""
+        import torch
+        self._check_device_tensor(alpha, "porosity")
+        torch.cuda.current_stream(alpha.device).synchronize()  # the library reads it on its own stream
+        j = c_double()
+        _check(load().lbm_adj_forward_device(self._h, c_void_p(alpha.data_ptr()), byref(j)))
+        return j.value
+
+    def backward_device(self, dJ: float = 1.0):
+        """backward() with the porosity gradient returned as a torch tensor on
+        the domain's GPU; (grad_alpha, grad_wall)."""
+        import torch
+        g = torch.empty((self.dom.zl, self.dom.yl, self.dom.xl), dtype=torch.float64,
+                        device=f"cuda:{self.dom.device}")
+        gw = np.zeros(3)
+        _check(load().lbm_adj_backward_device(self._h, dJ, c_void_p(g.data_ptr()), _dptr(gw)))
+        return g, gw
+
+    def _check_device_tensor(self, t, what):
+        n = self.dom.xl * self.dom.yl * self.dom.zl
+        if not (getattr(t, "is_cuda", False) and str(t.dtype) == "torch.float64" and t.is_contiguous()
+                and t.numel() == n and t.device.index == self.dom.device):
+            raise ValueError(f"{what}: expected a contiguous float64 tensor of {n} values on cuda:{self.dom.device}")
 
     def jvp(self, alpha, dalpha):
         a, p = self._alpha(alpha)

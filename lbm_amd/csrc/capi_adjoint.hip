@@ -74,6 +74,32 @@ int forward_step(lbm_domain* d, const void* src, void* dst, bool mat) {
     return LBM_OK;
 }
 
+// Device-resident porosity and gradients: interior arrays (x fastest) <->
+// the padded element layout, in the mode's arithmetic type for alpha.
+template <class C>
+__global__ void __launch_bounds__(256) k_alpha_scatter(FieldLayout L, const double* __restrict__ in, C* __restrict__ out) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= static_cast<int64_t>(L.xl) * L.yl * L.zl) return;
+    const int x = 1 + static_cast<int>(i % L.xl);
+    const int64_t r = i / L.xl;
+    const int y = 1 + static_cast<int>(r % L.yl), z = 1 + static_cast<int>(r / L.yl);
+    out[cell_elem(L, x, y, z)] = static_cast<C>(in[i]);
+}
+
+__global__ void __launch_bounds__(256) k_grad_gather(FieldLayout L, const double* __restrict__ g, double dJ,
+                                                     double* __restrict__ out) {
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= static_cast<int64_t>(L.xl) * L.yl * L.zl) return;
+    const int x = 1 + static_cast<int>(i % L.xl);
+    const int64_t r = i / L.xl;
+    const int y = 1 + static_cast<int>(r % L.yl), z = 1 + static_cast<int>(r / L.yl);
+    out[i] = dJ * g[cell_elem(L, x, y, z)];
+}
+
+uint32_t interior_blocks(const FieldLayout& L) {
+    return static_cast<uint32_t>((static_cast<int64_t>(L.xl) * L.yl * L.zl + 255) / 256);
+}
+
 int upload_alpha(lbm_domain* d, const double* alpha) {
     if (!d->d_alpha) return fail(LBM_ERR_STATE, "the domain has no Brinkman field (lbm_set_brinkman before finalize)");
     const std::vector<char> a = alpha_in_layout(d, alpha);
@@ -185,6 +211,39 @@ int lbm_adj_forward(lbm_tape* t, const double* alpha, double* J) {
     t->timing[3] = static_cast<double>(t->T);
     if (J) *J = out[0];
     t->forward_done = true;
+    return LBM_OK;
+}
+
+int lbm_adj_forward_device(lbm_tape* t, const double* d_alpha, double* J) {
+    if (!t) return fail(LBM_ERR_ARG, "null tape");
+    lbm_domain* d = t->d;
+    if (int rc = check_handle(d)) return rc;
+    if (d_alpha) {
+        if (!d->d_alpha)
+            return fail(LBM_ERR_STATE, "the domain has no Brinkman field (lbm_set_brinkman before finalize)");
+        if (d->mode == Mode::F64S)
+            hipLaunchKernelGGL(k_alpha_scatter<double>, dim3(interior_blocks(d->L)), dim3(256), 0, d->stream, d->L,
+                               d_alpha, static_cast<double*>(d->d_alpha));
+        else
+            hipLaunchKernelGGL(k_alpha_scatter<float>, dim3(interior_blocks(d->L)), dim3(256), 0, d->stream, d->L,
+                               d_alpha, static_cast<float*>(d->d_alpha));
+        HIPCHK(hipGetLastError());
+    }
+    return lbm_adj_forward(t, nullptr, J);
+}
+
+int lbm_adj_backward_device(lbm_tape* t, double dJ, double* d_grad_alpha, double grad_wall[3]) {
+    if (!t) return fail(LBM_ERR_ARG, "null tape");
+    lbm_domain* d = t->d;
+    if (int rc = check_handle(d)) return rc;
+    if (d_grad_alpha && !d->d_alpha) return fail(LBM_ERR_STATE, "no Brinkman field: no porosity gradient");
+    if (int rc = lbm_adj_backward(t, dJ, nullptr, grad_wall)) return rc;
+    if (d_grad_alpha) {
+        hipLaunchKernelGGL(k_grad_gather, dim3(interior_blocks(d->L)), dim3(256), 0, d->stream, d->L,
+                           t->d_grad_alpha, dJ, d_grad_alpha);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(d->stream));
+    }
     return LBM_OK;
 }
 

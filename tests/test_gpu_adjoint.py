@@ -128,6 +128,50 @@ def test_fp32_adjoint_close_to_fp64(small_problem):
     assert e < 1e-4
 
 
+@pytest.mark.parametrize("storage", ["f32", "f64s"])
+def test_torch_objective_on_device_equals_host(storage):
+    """torch.autograd through LbmObjective and the periodic trilinear map
+    (16^3 fine, 4^3 coarse, 24 steps). A coarse porosity on the GPU takes
+    the device-pointer entry points and never visits the host: its objective
+    and fine gradient equal the host entry points' on the same fine values
+    bit for bit, and its coarse gradient (torch's transpose of the map)
+    equals upsample_np_adjoint of the fine gradient to rounding, as does the
+    gradient of a coarse porosity on the CPU."""
+    import torch
+    n, nc, steps = 16, 4, 24
+    rng = np.random.default_rng(11)
+    ac = rng.uniform(0.1, 0.9, (nc, nc, nc))
+    target = 1e-3 * rng.standard_normal((n, n, 3))
+    d = periodic_brinkman(n, storage, autograd.upsample_np(ac, (n, n, n)), (1e-3, 0, 0))
+    tape = capi.Tape(d, steps, capi.OBJ_PLANE_VELOCITY_MISFIT, plane_x=n // 2, target_u=target, checkpoint_every=5)
+    obj = autograd.make_objective_function()
+    grads = {}
+    for dev in ("cuda:0", "cpu"):
+        c = torch.tensor(ac, dtype=torch.float64, device=dev, requires_grad=True)
+        fine = autograd.upsample_periodic(c, (n, n, n))
+        J = obj.apply(fine, tape)
+        J.backward()
+        grads[dev] = c.grad.detach().cpu().numpy()
+        if dev == "cuda:0":
+            fine_values = fine.detach().cpu().numpy()
+            J_dev = float(J.detach())
+            g_dev, _ = tape.backward_device(1.0)
+            g_dev = g_dev.cpu().numpy()
+        say(f"  {storage} {dev}: J {float(J.detach()):.15e}")
+    # the host entry points on the same fine porosity
+    J_host = tape.forward(fine_values)
+    g_host, _ = tape.backward(1.0)
+    assert J_dev == J_host
+    assert np.array_equal(g_dev.view(np.uint64), g_host.view(np.uint64))
+    ref = autograd.upsample_np_adjoint(g_host, ac.shape)
+    for dev, gc in grads.items():
+        err = np.abs(gc - ref).max() / np.abs(ref).max()
+        say(f"  {storage} {dev}: coarse gradient vs upsample_np_adjoint {err:.2e}")
+        assert err < 1e-9
+    tape.close()
+    d.close()
+
+
 def test_c4_inverse_problem_gradient():
     """D3Q19 128^3 fp32, porosity from a seeded uniform 32^3 grid (trilinear,
     periodic), g = 1e-5 x, tau 0.8, 2000 steps, sqrt(N) checkpoints; target
