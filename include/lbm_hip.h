@@ -367,17 +367,18 @@ int lbm_set_index_width(lbm_domain* d, int bits);
 
 /* Time steps per pass over the field. With planes under 4 GiB, lbm_step
  * runs two steps in one pass: the field is read once and written once for
- * both, with the intermediate state held on chip. This applies to bulk-only
- * domains (periodic along all three axes) in fp32, fp16 or bf16 storage
- * (16-bit modes: x rows longer than 32 cells), and to fp32 domains with
+ * both, with the intermediate state held on chip. This applies to fp32,
+ * fp16 and bf16 storage (16-bit modes: x rows longer than 32 cells), on
+ * bulk-only domains (periodic along all three axes) and on domains with
  * NoSlip, MovingWall, Inflow, Outflow and Pressure boundaries and solid
  * cells, where the first step after initialisation or lbm_set_pdf stays a
  * single step. A domain with a FreeSlip condition, a Brinkman field, the
  * in-place layout or a vector width set by lbm_set_tuning steps singly.
  * Results on fluid cells are the same bits as single steps. steps_per_pass:
  * 0 (default, automatic): two steps per pass where that measured faster,
- * which leaves D3Q27 domains with boundary cells stepping singly; 2: two
- * steps per pass wherever it applies; 1: single steps. z_chunk: planes per
+ * which leaves domains with boundary cells in D3Q27 or in 16-bit storage
+ * stepping singly; 2: two steps per pass wherever it applies; 1: single
+ * steps. z_chunk: planes per
  * block of the pass (0 = automatic). No reference counterpart: a scheduling
  * choice of this implementation. */
 int lbm_set_time_blocking(lbm_domain* d, int steps_per_pass, int z_chunk);

@@ -589,8 +589,8 @@ __device__ __forceinline__ bool near_cell(const StepArgs& a, int x, int y, int z
             f[q] = IO::template load<q>(src[q * Ly.plane + se]);
         } else {
             bool stale;
-            f[q] = IO::template scaled<q>(pull_boundary<L, M, q>(a, cond_at(a.conds, k - KIND_COND0), os, own, x, y,
-                                                                 z, sx, sy, sz, se, stale));
+            f[q] = IO::template through_storage<q>(pull_boundary<L, M, q>(a, cond_at(a.conds, k - KIND_COND0), os, own,
+                                                                          x, y, z, sx, sy, sz, se, stale));
         }
     });
 
@@ -660,7 +660,7 @@ __device__ __forceinline__ void near_rules(const StepArgs& a, typename ElemIdx<B
                 auto finv = [&](auto qc) { return own(std::integral_constant<int, Q - 1 - decltype(qc)::value>{}); };
                 auto put = [&](auto qc, Calc v) {
                     constexpr int q = decltype(qc)::value;
-                    f[q] = pick_bits(((lm >> q) & 1u) != 0, IO::template scaled<q>(v), f[q]);
+                    f[q] = pick_bits(((lm >> q) & 1u) != 0, IO::template through_storage<q>(v), f[q]);
                 };
                 if (ck == BC_MOVINGWALL) {
                     static_for<0, Q>([&](auto qc) {
@@ -689,8 +689,8 @@ __device__ __forceinline__ void near_rules(const StepArgs& a, typename ElemIdx<B
             const I se = elem_at<B32>(Ly, sx, sy, sz);
             const uint8_t sk = kind_at(a.kind, se);
             bool stale;
-            f[q] = IO::template scaled<q>(pull_boundary<L, M, q, FS, B32>(a, cond_at(a.conds, sk - KIND_COND0), os, own,
-                                                                           x, y, z, sx, sy, sz, se, stale));
+            f[q] = IO::template through_storage<q>(pull_boundary<L, M, q, FS, B32>(
+                a, cond_at(a.conds, sk - KIND_COND0), os, own, x, y, z, sx, sy, sz, se, stale));
         }
     });
 }

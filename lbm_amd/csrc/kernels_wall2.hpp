@@ -1,6 +1,6 @@
 // Two time steps per pass for domains with boundary cells: walls, solid
 // obstacles, inflow and outflow faces (NoSlip, MovingWall, Inflow, Outflow,
-// Pressure; fp32 storage). A domain with a FreeSlip condition steps singly,
+// Pressure; fp32, fp16 or bf16 storage). A domain with a FreeSlip condition steps singly,
 // since that rule reads cells other than the one it serves.
 //
 // The march is that of k_bulk2 (kernels_tb.hpp): a block owns a TX x TY
@@ -24,9 +24,11 @@
 //     rule value of (b, k) and writes it into x's own slot k. Nothing else
 //     writes that slot: its only regular sender would be b. Where all links
 //     share one condition (lcond) its data is loaded once for all of them.
-// The storage format is fp32, the arithmetic type, so a rule value written
-// to memory or LDS is the value the single step computes, and a pass equals
-// two single steps bit for bit on every fluid cell.
+// A rule value crosses memory or LDS in the storage format. The single step
+// takes its rule values through the storage format too (Io::through_storage:
+// as the reference's boundary pass stores them in the field), so a pass
+// equals two single steps bit for bit on every fluid cell, in fp32, fp16 and
+// bf16 storage.
 //
 // Coordinates along periodic axes wrap as in k_bulk2; along walled axes the
 // ring reaches the ghost layer (0 and l + 1), whose cells are not fluid.
@@ -121,7 +123,6 @@ __global__ void __launch_bounds__(256) k_wall2(StepArgs a, int ntx, int nty, int
     using Calc = typename IO::Calc;
     using S = TbSlots<L>;
     constexpr int Q = L::Q;
-    static_assert(sizeof(T) == sizeof(Calc), "boundary values cross memory in the arithmetic type");
     static_assert(TB_PASS == TB_OUT + 1 && TB_NP - TB_CELLS <= 256, "tile cells on k < TB_OUT, the ring on the last k");
     __shared__ T lds[S::NSLOT][TB_CELLS];
 

@@ -96,7 +96,10 @@ struct OpsImpl {
 
     // the walled pass: fp32 storage (boundary values cross memory and LDS in
     // the arithmetic type) whose LDS slots fit 64 KiB
-    static constexpr bool wall2_built() { return M == Mode::F32 && tb_lds_bytes<L, M>() <= 65536; }
+    static constexpr bool wall2_built() {
+        return (M == Mode::F32 || M == Mode::F16 || M == Mode::BF16) &&
+               static_cast<size_t>(TbSlots<L>::NSLOT) * TB_CELLS * sizeof(typename Io<M, L>::T) <= 65536;
+    }
     template <int RULES>
     static void launch_wall2(const StepArgs& a, const StepPlan& p, hipStream_t s) {
         if (a.nb > 0) {

@@ -135,6 +135,15 @@ struct Io {
     __device__ __forceinline__ static T from_calc(Calc c) {
         return store<q>(scaled<q>(c));
     }
+    // A boundary rule's value g as the pull reads it: written into the
+    // boundary cell in the storage format, then loaded (the reference's
+    // boundary pass stores its values in the field before the stream pulls
+    // them). An identity for fp32 and fp64 storage; for 16-bit storage it
+    // rounds, in every kernel alike.
+    template <int q>
+    __device__ __forceinline__ static Calc through_storage(Calc g) {
+        return load<q>(from_calc<q>(g));
+    }
 };
 
 }  // namespace lbm_amd

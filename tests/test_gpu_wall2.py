@@ -1,8 +1,8 @@
 """Two time steps per pass on domains with boundary cells (k_emit_rules +
 k_wall2) against single steps.
 
-With fp32 storage, walls, solid obstacles and inflow/outflow/pressure faces
-step two steps per pass by default: before the pass every boundary link gets
+With fp32 storage (by default) or fp16/bf16 storage (on request), walls,
+solid obstacles and inflow/outflow/pressure faces step two steps per pass: before the pass every boundary link gets
 its rule value from the stored values of the fluid cell it serves, step 1
 pulls plainly, and a cell next to a non-fluid cell leaves the rule values its
 step 2 needs, computed from its own step-1 values, in its own LDS slots. The
@@ -12,7 +12,9 @@ lbm_get_pdf reports it (it writes the boundary links' rule values itself):
 tiles cut by walls on every face, partial tiles, solid spheres, inflow,
 outflow, pressure and moving walls, periodic axes mixed with walls, odd step
 counts, several z chunk sizes, the reference's custom-extent scenario and the
-256^3 cavity. FreeSlip, Brinkman and 16-bit walled domains keep single steps.
+256^3 cavity. The rule values cross memory in the storage format in both
+forms (the single step rounds them through it too), so 16-bit storage is
+bitwise as well. FreeSlip, Brinkman and fp64 walled domains keep single steps.
 """
 import sys
 from pathlib import Path
@@ -65,7 +67,7 @@ def check(shape, q, storage, tau, sc, steps, periodic=(0, 0, 0), mask=None, chun
     assert np.abs(u1).max() > 1e-5, label
 
 
-@pytest.mark.parametrize("storage", ["f32"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
 @pytest.mark.parametrize("q", [15, 19, 27])
 def test_cavity_pass_equals_single_steps(q, storage):
     for shape in ((70, 19, 13), (40, 9, 5), (130, 17, 33)):
@@ -74,7 +76,7 @@ def test_cavity_pass_equals_single_steps(q, storage):
         check(shape, q, storage, 0.6, sc, 9, chunks=(0, 1, 3), label=f"cavity {shape}")
 
 
-@pytest.mark.parametrize("storage", ["f32"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
 @pytest.mark.parametrize("q", [19, 27])
 def test_sphere_channel_pass_equals_single_steps(q, storage):
     shape = (96, 40, 36)
@@ -111,7 +113,7 @@ def test_reference_step_scenario_pass_equals_single_steps(q):
     check((sc.xl, sc.yl, sc.zl), q, "f32", 0.6, sc, 40, label="step.xml")
 
 
-def test_freeslip_brinkman_and_16bit_walled_domains_step_singly():
+def test_freeslip_brinkman_and_fp64_walled_domains_step_singly():
     sc = scenario.parse_scenario_file(GOLDEN / "shearflow.xml")
     d = capi.Domain(sc.xl, sc.yl, sc.zl, 0.6, q=19, storage="f32")
     sc.apply(d)
@@ -144,15 +146,27 @@ def test_freeslip_brinkman_and_16bit_walled_domains_step_singly():
     with pytest.raises(capi.LbmError):
         d.set_time_blocking(-1)
     d.close()
-    for storage in ("f16", "bf16", "f64", "f64s"):
+    # 16-bit storage: single steps by default (measured faster), the pass on
+    # request; fp64 has no walled pass
+    for storage, forced in (("f16", 2), ("bf16", 2), ("f64", 1), ("f64s", 1)):
         d = capi.Domain(*shape, 0.6, q=19, storage=storage)
         scenario.cavity(*shape).apply(d)
         d.finalize()
         assert d.info()["steps_per_pass"] == 1, storage
+        d.set_time_blocking(2)
+        assert d.info()["steps_per_pass"] == forced, storage
         d.close()
+    # 16-bit rows of 32 cells or fewer step singly even on request (as
+    # bulk-only domains do)
+    d = capi.Domain(32, 12, 10, 0.6, q=19, storage="f16")
+    scenario.cavity(32, 12, 10).apply(d)
+    d.finalize()
+    d.set_time_blocking(2)
+    assert d.info()["steps_per_pass"] == 1
+    d.close()
 
 
-@pytest.mark.parametrize("storage", ["f32"])
+@pytest.mark.parametrize("storage", ["f32", "f16"])
 def test_cavity_256_pass_equals_single_steps(storage):
     """The benchmark's walled case (256^3 lid-driven cavity, D3Q19, tau 0.6):
     41 steps (the first single, then 20 passes) equal 41 single steps bit for
