@@ -60,6 +60,7 @@ struct BatchArgs {
     int materialised;  // the step pulls stored boundary values
     int freeslip;      // some domain has a FreeSlip condition (cell tiles built with its rule)
     int idx32;         // every domain's plane is under 4 GiB (cell tiles use 32-bit byte offsets)
+    int nxcd;          // L2 domains the launch's blocks are dealt to (cell tiles: tile_walk)
     // one flag per domain, set to 1 by a step in which a fluid cell of the
     // domain read a NaN or infinite value (never cleared by the kernels)
     int* status;
@@ -73,5 +74,38 @@ struct BatchOps {
 };
 
 const BatchOps* batch_ops(int q, Mode m);
+
+// Tiles of one wavefront in the cell-tile kernels: t, t + stride, ... below
+// end. The hardware deals the blocks of a launch to its nxcd L2 domains in
+// turn (block b to domain b mod nxcd). Each domain's wavefronts walk one
+// contiguous range of the tile list, sized by its share of the wavefronts,
+// so tiles that share halo rows and planes (neighbours in the list, of one
+// batched domain) are fetched into one L2. The ranges partition the list,
+// so every tile is stepped exactly once. nxcd <= 1: the plain walk first,
+// first + (wavefronts in the grid), ... Requires gridDim.x * blockDim.x / 64
+// * ntiles < 2^63 and ntiles < 2^30.
+struct TileWalk {
+    int t, end, stride;
+};
+
+__device__ __forceinline__ TileWalk tile_walk(int64_t ntiles, int nxcd) {
+    const int wpb = static_cast<int>(blockDim.x) >> 6;
+    const int g = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+    int slot = b, nslots = g, before = 0;  // before: blocks dealt to the domains ahead of this one
+    if (nxcd > 1) {
+        const int x = b % nxcd, full = g / nxcd, rem = g % nxcd;
+        slot = b / nxcd;
+        nslots = full + (x < rem ? 1 : 0);
+        before = x * full + (x < rem ? x : rem);
+    }
+    const int64_t waves = static_cast<int64_t>(g) * wpb;
+    const int64_t lo = ntiles * (static_cast<int64_t>(before) * wpb) / waves;
+    const int64_t hi = ntiles * (static_cast<int64_t>(before + nslots) * wpb) / waves;
+    TileWalk w;
+    w.t = __builtin_amdgcn_readfirstlane(static_cast<int>(lo) + slot * wpb + (static_cast<int>(threadIdx.x) >> 6));
+    w.end = static_cast<int>(hi);
+    w.stride = nslots * wpb;
+    return w;
+}
 
 }  // namespace lbm_amd

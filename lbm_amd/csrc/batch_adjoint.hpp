@@ -42,6 +42,7 @@ struct BatchAdjArgs {
     int terminal;
     int materialised;
     int idx32;  // every domain's plane is under 4 GiB (32-bit byte offsets)
+    int nxcd;   // L2 domains the launch's blocks are dealt to (tile_walk)
 };
 
 struct BatchAdjOps {

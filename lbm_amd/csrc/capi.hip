@@ -174,7 +174,7 @@ int make_plan(lbm_domain* d) {
     p.tb_zc = std::max(1, std::min(zc, d->L.zl));
     // gfx950 deals blocks in turn to its XCDs (32 CUs and one L2 each);
     // grouping neighbouring tiles on one XCD lets them share halo lines in L2
-    p.tb_nxcd = (ncu >= 32 && ncu % 32 == 0) ? ncu / 32 : 1;
+    p.tb_nxcd = xcd_count(ncu);
     p.tb_nblocks = tiles * ((d->L.zl + p.tb_zc - 1) / p.tb_zc);
     // (byte offsets within a plane are 32-bit in the pass.) The 16-bit pass
     // is bound by instruction issue: on rows of 32 cells or fewer most lanes

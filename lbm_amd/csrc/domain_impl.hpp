@@ -113,6 +113,10 @@ inline size_t elem_size(Mode m) {
 
 inline int64_t roundup(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
+// L2 domains (XCDs) of a gfx950 device with ncu compute units: 32 CUs and
+// one L2 each; the hardware deals a launch's blocks to them in turn.
+inline int xcd_count(int ncu) { return (ncu >= 32 && ncu % 32 == 0) ? ncu / 32 : 1; }
+
 }  // namespace lbm_amd
 
 // internal header: the domain struct below is spelled with these names

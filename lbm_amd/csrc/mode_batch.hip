@@ -56,17 +56,15 @@ __global__ void __launch_bounds__(256) k_batch_bulk(BatchArgs b) {
     }
 }
 
-// Cell tiles: wavefront w walks tiles w, w + (waves in the grid), ...; each
-// lane takes one cell of its tile.
+// Cell tiles: each wavefront walks its tiles (tile_walk, one L2 domain's
+// range of the list); each lane takes one cell of its tile.
 // The tile index is wavefront-uniform; readfirstlane states it, so the tile
 // and the domain view are read into scalar registers once per wavefront.
 template <class L, Mode M, bool FS, bool B32>
 __global__ void __launch_bounds__(256) k_batch_cells(BatchArgs b) {
     const int lane = threadIdx.x & 63;
-    const int nwaves = static_cast<int>(gridDim.x) * (static_cast<int>(blockDim.x) >> 6);
-    const int first = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * (blockDim.x >> 6) +
-                                                                      (threadIdx.x >> 6)));
-    for (int t = first; t < b.ntiles; t += nwaves) {
+    const TileWalk w = tile_walk(b.ntiles, b.nxcd);
+    for (int t = w.t; t < w.end; t += w.stride) {
         const TileRef tr = b.tiles[t];
         const BatchView& v = b.views[tr.dom];
         const int z = 1 + tr.local / v.tpp;

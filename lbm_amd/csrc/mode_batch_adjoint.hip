@@ -44,10 +44,8 @@ template <class L, Mode M, bool B32>
 __global__ void __launch_bounds__(256) k_badj_recompute(BatchAdjArgs b) {
     using T = typename StorageOf<M>::T;
     const int lane = threadIdx.x & 63;
-    const int nwaves = static_cast<int>(gridDim.x) * (static_cast<int>(blockDim.x) >> 6);
-    const int first = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * (blockDim.x >> 6) +
-                                                                      (threadIdx.x >> 6)));
-    for (int t = first; t < b.ntiles; t += nwaves) {
+    const TileWalk w = tile_walk(b.ntiles, b.nxcd);
+    for (int t = w.t; t < w.end; t += w.stride) {
         const TileRef tr = b.tiles[t];
         const BatchAdjView& v = b.views[tr.dom];
         int x, y, z;
@@ -74,10 +72,8 @@ __global__ void __launch_bounds__(256) k_badj_backward(BatchAdjArgs b) {
     using T = typename StorageOf<M>::T;
     static_assert(sizeof(T) == sizeof(typename StorageOf<M>::Calc), "mu slots hold the arithmetic type");
     const int lane = threadIdx.x & 63;
-    const int nwaves = static_cast<int>(gridDim.x) * (static_cast<int>(blockDim.x) >> 6);
-    const int first = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * (blockDim.x >> 6) +
-                                                                      (threadIdx.x >> 6)));
-    for (int t = first; t < b.ntiles; t += nwaves) {
+    const TileWalk w = tile_walk(b.ntiles, b.nxcd);
+    for (int t = w.t; t < w.end; t += w.stride) {
         const TileRef tr = b.tiles[t];
         const BatchAdjView& v = b.views[tr.dom];
         int x, y, z;
