@@ -16,6 +16,12 @@ HIP-event-timed average launch; beside it the HBM bytes rocprofv3 measured
 per launch), the fp16/bf16 modes and a CPU baseline (the fp64 CPU oracle,
 OpenMP on the host cores, bounded sample).
 Further legs ride on the same line:
+- "walled": the 256^3 lid-driven cavity (D3Q19, tau 0.6) and a 384x192x192
+  channel around seeded spheres (D3Q27, tau 0.55), fp32 and fp16, each as
+  two steps per pass (k_wall2) and as single steps (k_cells), with the form
+  lbm_step picks by default named;
+- "report": the periodic box in fp64 (the reference's operation order) and
+  in D3Q15 / D3Q27;
 - "inplace_aa": the same workload in the in-place (AA) layout, one buffer;
 - "adjoint": the inverse problem on a periodic 128^3 D3Q19 fp32 box with a
   Brinkman porosity field (seeded uniform 32^3, trilinear), body force 1e-5 x,
@@ -24,9 +30,10 @@ Further legs ride on the same line:
   sweep, and their ratio;
 - "batch": 512 lid-driven D3Q19 fp32 cavities (edges log-uniform in
   [16, 96], Re uniform in [10, 1000] through per-domain tau) stepped 2000
-  times in one persistent, graph-captured launch per step after autotuning,
-  then the batched adjoint of every cavity's kinetic energy with respect to
-  its lid velocity over 100 steps.
+  times in one graph-captured launch per step after autotuning, then the
+  batched adjoint of every cavity's kinetic energy with respect to its lid
+  velocity over 2000 steps (two checkpoint levels).
+Progress lines (one per finished leg) go to stderr.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-adjoint] [--no-batch]
                     [--modes f32,f16,bf16]
@@ -397,6 +404,10 @@ def main() -> None:
               "(multi-GPU slab decomposition is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    def progress(msg):
+        # one line per finished leg on stderr (stdout carries the JSON line)
+        print(f"bench: {msg}", file=sys.stderr, flush=True)
+
     from lbm_amd import capi
     capi.load()
     # plain 1 GiB device copy: achievable-bandwidth reference, and the kernel of
@@ -409,6 +420,7 @@ def main() -> None:
     modes = [m for m in a.modes.split(",") if m]
     res = {m: run_mode(capi, m, a.steps, a.warmup, rho, u) for m in modes}
     single = {m: run_mode(capi, m, a.steps, a.warmup, rho, u, steps_per_pass=1) for m in modes}
+    progress(f"periodic 256^3 legs done ({', '.join(modes)})")
     head = res.get("f32") or res[modes[0]]
     head_mode = "f32" if "f32" in res else modes[0]
     traffic = traffic_from_profiles(head_mode, "pass" if head["steps_per_pass"] == 2 else "bulk")
@@ -466,12 +478,17 @@ def main() -> None:
         line["f16_over_f32_single_step"] = round(single["f16"]["mlups"] / single["f32"]["mlups"], 3)
     if not a.no_walled:
         line["walled"] = walled_leg(capi, min(a.steps, 200), [m for m in a.walled_modes.split(",") if m])
+        progress("walled leg done")
     if not a.no_report:
         line["report"] = report_leg(capi, min(a.steps, 100), a.warmup, rho, u)
+        progress("report rows done")
     if not a.no_aa:
         line["inplace_aa"] = aa_leg(capi, head_mode, min(a.steps, 200), rho, u)
+        progress("in-place layout leg done")
     line["adjoint"] = None if a.no_adjoint else adjoint_leg(capi)
+    progress("adjoint leg done")
     line["batch"] = None if a.no_batch else batch_leg(capi, a.batch_steps, adj_steps=a.batch_adjoint_steps)
+    progress("batch leg done")
     line["cpu_baseline"] = None if a.no_cpu else cpu_baseline(rho, u, a.cpu_steps)
     print(json.dumps(line), flush=True)
 
