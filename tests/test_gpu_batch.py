@@ -71,6 +71,35 @@ def test_batch_equals_single(storage, config):
         assert np.array_equal(bits(db.get_pdf()), bits(ds.get_pdf()))
 
 
+@pytest.mark.parametrize("vec", [2, 4])
+@pytest.mark.parametrize("config", ["rows_separate", "rows_fused"])
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_batch_rows_16bit_equals_single(storage, config, vec):
+    """The row kernels of 16-bit storage (2 or 4 cells per thread), on
+    cavities whose rows are long enough to be line-aligned (126 cells and
+    more; shorter rows are packed and take the cell tiles): the same bits as
+    each domain stepped alone."""
+    specs = [(130, 10, 8, 0.62), (150, 7, 12, 0.55), (200, 5, 6, 0.7), (127, 3, 4, 0.9)]
+    say(f"batch rows == single {storage} {config} vec={vec}: {specs}")
+    batch_doms = [cavity(*s, storage) for s in specs]
+    single_doms = [cavity(*s, storage) for s in specs]
+    b = capi.Batch(batch_doms)
+    b.set_config(KERNELS[config], vec, 0, 8)
+    info = b.info()
+    assert info["kernel"] == KERNELS[config] and info["vec"] == vec, info
+    b.step(23)
+    b.sync()
+    for d in single_doms:
+        d.step(23)
+    for db, ds in zip(batch_doms, single_doms):
+        assert np.array_equal(bits(db.get_pdf()), bits(ds.get_pdf()))
+        _, u = db.moments()
+        assert np.abs(u).max() > 1e-4
+    b.close()
+    for d in batch_doms + single_doms:
+        d.close()
+
+
 def test_batch_subsample_against_oracle():
     specs = workloads.cavity_set(n=24, lmin=16, lmax=48, re_min=10, re_max=100, seed=11)
     say("batch vs oracle")
