@@ -33,8 +33,14 @@ KERNELS = {"rows_separate": capi.BATCH_ROWS_SEPARATE, "rows_fused": capi.BATCH_R
            "cells": capi.BATCH_CELLS}
 
 
-@pytest.mark.parametrize("config", ["rows_separate", "rows_fused", "cells", "tuned"])
-@pytest.mark.parametrize("storage", ["f32", "f16", "f64s"])
+# These cavities are short in x, so their rows are packed and the row kernels
+# take one cell per thread, which 16-bit storage does not build (its row
+# forms are covered on long rows by test_batch_rows_16bit_equals_single).
+FORMS = [(s, c) for s in ("f32", "f16", "f64s") for c in ("rows_separate", "rows_fused", "cells", "tuned")
+         if not (s == "f16" and c.startswith("rows"))]
+
+
+@pytest.mark.parametrize("storage,config", FORMS)
 def test_batch_equals_single(storage, config):
     """Every kernel form (row tiles with the near-boundary cells in a second
     launch or in the same pass, 64-cell tiles per wavefront) and the
@@ -44,11 +50,7 @@ def test_batch_equals_single(storage, config):
     batch_doms = [cavity(*s[:4], storage) for s in specs]
     single_doms = [cavity(*s[:4], storage) for s in specs]
     b = capi.Batch(batch_doms)
-    # these cavities are short in x, so their rows are packed and the row
-    # kernels take one cell per thread (not built for 16-bit storage)
     vec = 1
-    if config.startswith("rows") and storage == "f16":
-        pytest.skip("row kernels of 16-bit storage need rows of at least 126 cells")
     if config == "tuned":
         say(f"  tune: {b.tune(2):.3f} ms/step, {b.info()}")
     else:
