@@ -376,9 +376,8 @@ int lbm_set_index_width(lbm_domain* d, int bits);
  * in-place layout or a vector width set by lbm_set_tuning steps singly.
  * Results on fluid cells are the same bits as single steps. steps_per_pass:
  * 0 (default, automatic): two steps per pass where that measured faster,
- * which leaves domains with boundary cells in D3Q27 or in 16-bit storage
- * stepping singly; 2: two steps per pass wherever it applies; 1: single
- * steps. z_chunk: planes per
+ * which leaves domains with boundary cells in 16-bit storage stepping
+ * singly; 2: two steps per pass wherever it applies; 1: single steps. z_chunk: planes per
  * block of the pass (0 = automatic). No reference counterpart: a scheduling
  * choice of this implementation. */
 int lbm_set_time_blocking(lbm_domain* d, int steps_per_pass, int z_chunk);

@@ -181,14 +181,13 @@ int make_plan(lbm_domain* d) {
     // of its 64-cell tile rows idle, and single steps (cell tiles over packed
     // rows) measured faster there.
     p.pairs = pass_ok && o->step2 != nullptr && !walled && p.has_bulk && p.tb_nblocks <= 0x7fffffffLL;
-    // Automatic choice (tune_pass 0): the walled pass on D3Q15 and D3Q19 with
-    // fp32 storage. On D3Q27 its step 1 holds 27 values per patch cell across
-    // the plane and measured slower than single steps (Q27 channel around a
-    // sphere, fp32); with 16-bit storage it is bound by instruction issue and
-    // measured slower than single steps (256^3 cavity D3Q19: 0.39 against
-    // 0.29 ms per step).
+    // Automatic choice (tune_pass 0): the walled pass with fp32 storage (256^3
+    // cavity D3Q19: 0.36 against 0.50 ms per step; 384x192x192 channel around
+    // spheres D3Q27: 0.80 against 0.82). With 16-bit storage it is bound by
+    // instruction issue and measured slower than single steps (the cavity
+    // in fp16: 0.39 against 0.28 ms per step).
     p.wpairs = pass_ok && o->step2w != nullptr && walled && d->tune_vec == 0 && !p.freeslip && !p.brink &&
-               d->n_fluid > 0 && (d->tune_pass == 2 || (o->q <= 19 && d->esize == 4)) &&
+               d->n_fluid > 0 && (d->tune_pass == 2 || d->esize == 4) &&
                p.tb_nblocks <= 0x7fffffffLL;
     return LBM_OK;
 }

@@ -62,7 +62,33 @@ __global__ void __launch_bounds__(256) k_emit_rules(StepArgs a) {
     T* __restrict__ buf = static_cast<T*>(a.dst);
     const uint32_t e = elem_at<true>(Ly, c.x, c.y, c.z);
     const uint32_t lm = glob(a.lmask)[e];
+    if (lm == 0) return;
     const int lc = glob(a.lcond)[e];
+    if (lc != LCOND_MIXED && cond_at(a.conds, lc).kind == BC_NOSLIP) {
+        // Every link NoSlip: the rule value of link q is the cell's own value
+        // of direction q-bar, so only the linked directions are read (a cell
+        // on an x wall has Q values in Q different lines, of which the other
+        // kinds' rules need all for the density). An unlinked direction reads
+        // the first linked one's element instead, a line fetched anyway, so
+        // that every load goes out unconditionally ahead of the masked stores.
+        const int q0 = __builtin_ctz(lm);
+        T v[Q];
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            v[q] = plane_ref<true>(buf, ((lm >> q) & 1u) ? Q - 1 - q : Q - 1 - q0, Ly.plane, e);
+        });
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if (lm & (1u << q)) {
+                const uint32_t se = elem_at<true>(Ly, wrap_if(c.x - L::cx(q), Ly.xl, Ly.per[0]),
+                                                  wrap_if(c.y - L::cy(q), Ly.yl, Ly.per[1]),
+                                                  wrap_if(c.z - L::cz(q), Ly.zl, Ly.per[2]));
+                plane_ref<true>(buf, q, Ly.plane, se) =
+                    IO::template from_calc<q>(IO::template to_calc<Q - 1 - q>(v[q]));
+            }
+        });
+        return;
+    }
     Calc own[Q];
     OwnAcc<L, Calc> acc;
     static_for<0, Q>([&](auto qc) {
